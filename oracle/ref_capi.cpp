@@ -1,0 +1,147 @@
+// ref_capi.cpp -- C-ABI wrapper over the reference's own decode sources, built
+// only into oracle/_ref/libvbtref.so (test infrastructure, never shipped).
+//
+// This file contains no decode logic: every entry point forwards to the
+// reference function named in its comment, catches its exceptions and returns
+// the message.  It lets the tests check the oracle restatement against the
+// real reference implementation on the same bytes.
+#include "render_temporal_decode.h"
+#include "render_temporal_payload.h"
+#include "scientific_decode.h"
+#include "vbt_file.h"
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <string>
+
+namespace {
+thread_local std::string g_err;
+int fail(const std::exception& e)
+{
+    g_err = e.what();
+    return 1;
+}
+} // namespace
+
+extern "C" {
+
+const char* vbtref_last_error() { return g_err.c_str(); }
+
+// src/render_temporal_decode.cpp:167
+float vbtref_half_to_float(uint16_t b) { return vbt::renderTemporalHalfBitsToFloat(b); }
+// src/render_temporal_decode.cpp:135
+uint16_t vbtref_float_to_half(float v) { return vbt::renderTemporalFloatToHalfBits(v); }
+
+// src/render_temporal_decode.cpp:304
+int vbtref_decode_control_value(const uint8_t* stream, uint32_t codec, uint16_t controlCount, uint8_t bins,
+                                uint32_t keyframeCount, uint32_t streamBytes, int totalFrames,
+                                uint16_t controlIndex, int frameIndex, float* out)
+{
+    try {
+        *out = vbt::decodeRenderTemporalControlValue(stream, static_cast<vbt::RenderTemporalSequenceCodec>(codec),
+                                                     controlCount, bins, 0, keyframeCount, streamBytes, true,
+                                                     totalFrames, controlIndex, frameIndex);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// src/render_temporal_payload.cpp:234 -- fields flattened into 24 u32
+int vbtref_parse_leaf(const uint8_t* leaf, uint64_t bytes, uint32_t* o)
+{
+    try {
+        const auto v = vbt::parseRenderTemporalPackedLeaf(leaf, bytes, true);
+        o[0] = static_cast<uint32_t>(v.header.mode);
+        o[1] = static_cast<uint32_t>(v.header.coarseCodec);
+        o[2] = static_cast<uint32_t>(v.header.fineCodec);
+        o[3] = v.header.coarseResolution;
+        o[4] = v.header.fineResolution;
+        o[5] = v.prefix.coarseKeyframeCount;
+        o[6] = v.prefix.fineKeyframeCount;
+        o[7] = v.layout.coarseOffset;
+        o[8] = v.layout.coarse.totalBytes;
+        o[9] = v.layout.fineOffset;
+        o[10] = v.layout.fine.descriptorOffset;
+        o[11] = v.layout.fine.descriptorBytes;
+        o[12] = v.layout.fine.totalBytes;
+        o[13] = v.layout.totalBytes;
+        o[14] = v.shellActiveVoxelCount;
+        o[15] = v.shellResidualOffset;
+        o[16] = v.shellResidualBytes;
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// src/render_temporal_decode.cpp:358 (after parse, as every caller does)
+int vbtref_leaf_control_value(const uint8_t* leaf, uint64_t bytes, int useFine, int totalFrames,
+                              uint16_t controlIndex, int frameIndex, float* out)
+{
+    try {
+        const auto v = vbt::parseRenderTemporalPackedLeaf(leaf, bytes, true);
+        *out = vbt::decodeRenderTemporalLeafControlValue(leaf, v.header, v.prefix, v.layout, useFine != 0, true,
+                                                         totalFrames, controlIndex, frameIndex);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// src/render_temporal_decode.cpp:404
+int vbtref_shell_voxel_value(const uint8_t* leaf, uint64_t bytes, int totalFrames, uint16_t voxel, int frameIndex,
+                             float* out)
+{
+    try {
+        const auto v = vbt::parseRenderTemporalPackedLeaf(leaf, bytes, true);
+        *out = vbt::decodeRenderTemporalLeafShellVoxelValue(leaf, v, true, totalFrames, voxel, frameIndex);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// render/src/vbt_file.cpp:174 -- returns an opaque handle or null
+void* vbtref_load_file(const char* path)
+{
+    auto* f = new vbt::render::VbtFile();
+    std::string err;
+    if (!vbt::render::loadVbtFile(path, *f, err)) {
+        g_err = err;
+        delete f;
+        return nullptr;
+    }
+    return f;
+}
+
+void vbtref_free_file(void* h) { delete static_cast<vbt::render::VbtFile*>(h); }
+
+// render/src/scientific_decode.cpp:349
+int vbtref_sci_value(void* h, uint32_t x, uint32_t y, uint32_t z, uint32_t t, float* out)
+{
+    try {
+        const auto& f = *static_cast<vbt::render::VbtFile*>(h);
+        vbt::render::Query4D q{x, y, z, t, vbt::render::leafIndexForVoxel(f.header, x, y, z)};
+        *out = vbt::render::decodeScientificValueAtCpu(f, q);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// render/src/scientific_decode.cpp:375
+int vbtref_sci_frame(void* h, uint32_t t, float* out)
+{
+    try {
+        const auto& f = *static_cast<vbt::render::VbtFile*>(h);
+        const auto frame = vbt::render::reconstructScientificFrameCpu(f, t);
+        std::memcpy(out, frame.data(), frame.size() * sizeof(float));
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+} // extern "C"

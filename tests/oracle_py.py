@@ -1,0 +1,240 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and, when built, of
+the reference decode compiled in oracle/_ref/libvbtref.so.
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it, as the checker / CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libvbtref.so")
+
+u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class Header(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint8) for n in ("mode", "coarseCodec", "fineCodec", "coarseResolution", "fineResolution")]
+
+
+class StreamLayout(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("descriptorOffset", "descriptorBytes", "binIndexOffset",
+                                                 "binIndexBytes", "keyframeOffset", "keyframeBytes", "totalBytes")]
+
+
+class LeafLayout(ctypes.Structure):
+    _fields_ = [("headerBytes", ctypes.c_uint32), ("prefixOffset", ctypes.c_uint32), ("prefixBytes", ctypes.c_uint32),
+                ("coarseOffset", ctypes.c_uint32), ("coarse", StreamLayout), ("fineOffset", ctypes.c_uint32),
+                ("fine", StreamLayout), ("totalBytes", ctypes.c_uint32)]
+
+
+class LeafView(ctypes.Structure):
+    _fields_ = [("header", Header), ("coarseKeyframeCount", ctypes.c_uint16), ("fineKeyframeCount", ctypes.c_uint16),
+                ("layout", LeafLayout), ("shellActiveVoxelCount", ctypes.c_uint16),
+                ("shellResidualOffset", ctypes.c_uint32), ("shellResidualBytes", ctypes.c_uint32)]
+
+
+class Trace(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_uint32), ("hi", ctypes.c_uint32), ("f0", ctypes.c_int32), ("f1", ctypes.c_int32),
+                ("bin", ctypes.c_uint32), ("keyStart", ctypes.c_uint32), ("keyCount", ctypes.c_uint32),
+                ("kind", ctypes.c_uint32)]
+
+
+class FileHeader(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("version", "width", "height", "depth", "frames", "leafSize",
+                                                 "coarseResolution", "maxCoarseKeep", "leafCount", "profileType",
+                                                 "coarseAcScaleCount")]
+
+
+class VFile(ctypes.Structure):
+    _fields_ = [("header", FileHeader), ("coarseAcScales", ctypes.c_void_p), ("offsetsWords", ctypes.c_void_p),
+                ("payloadWords", ctypes.c_void_p), ("payloadWordCount", ctypes.c_uint32)]
+
+
+class Compose(ctypes.Structure):
+    _fields_ = [("signRule", ctypes.c_int), ("background", ctypes.c_float), ("shellEmptyScale", ctypes.c_float),
+                ("shellEmptyMax", ctypes.c_float)]
+
+
+_O = None
+_R = None
+
+
+def oracle():
+    global _O
+    if _O is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.vbto_last_error.restype = ctypes.c_char_p
+        L.vbto_half_bits_to_float.restype = ctypes.c_float
+        L.vbto_half_bits_to_float.argtypes = [ctypes.c_uint16]
+        L.vbto_float_to_half_bits.restype = ctypes.c_uint16
+        L.vbto_float_to_half_bits.argtypes = [ctypes.c_float]
+        L.vbto_pack_header.restype = ctypes.c_uint32
+        L.vbto_unpack_header.argtypes = [ctypes.c_uint32, ctypes.POINTER(Header)]
+        L.vbto_compute_leaf_layout.argtypes = [ctypes.POINTER(Header), ctypes.c_uint16, ctypes.c_uint16, ctypes.c_int,
+                                               ctypes.POINTER(LeafLayout)]
+        L.vbto_parse_leaf.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(LeafView)]
+        L.vbto_decode_control_value.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint16, ctypes.c_uint8,
+                                                ctypes.c_uint8, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_uint16, ctypes.c_int,
+                                                ctypes.POINTER(ctypes.c_float), ctypes.POINTER(Trace)]
+        L.vbto_decode_leaf_control_value.argtypes = [ctypes.c_void_p, ctypes.POINTER(LeafView), ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_int, ctypes.c_uint16, ctypes.c_int,
+                                                     ctypes.POINTER(ctypes.c_float), ctypes.POINTER(Trace)]
+        L.vbto_decode_shell_voxel_value.argtypes = [ctypes.c_void_p, ctypes.POINTER(LeafView), ctypes.c_int,
+                                                    ctypes.c_int, ctypes.c_uint16, ctypes.c_int,
+                                                    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint32)]
+        L.vbto_load_file_memory.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(VFile)]
+        L.vbto_leaf_index_for_voxel.restype = ctypes.c_uint32
+        L.vbto_leaf_index_for_voxel.argtypes = [ctypes.POINTER(FileHeader), ctypes.c_uint32, ctypes.c_uint32,
+                                                ctypes.c_uint32]
+        L.vbto_decode_frame_dense.argtypes = [ctypes.POINTER(VFile), ctypes.c_int, ctypes.POINTER(Compose),
+                                              ctypes.c_void_p, ctypes.c_int]
+        L.vbto_query_points.argtypes = [ctypes.POINTER(VFile), ctypes.c_void_p, ctypes.c_uint64,
+                                        ctypes.POINTER(Compose), ctypes.c_void_p, ctypes.c_int]
+        L.vbto_page_table.restype = ctypes.c_uint32
+        L.vbto_page_table.argtypes = [ctypes.POINTER(VFile), ctypes.c_void_p]
+        L.vbto_leaf_cache_count.restype = ctypes.c_uint32
+        L.vbto_leaf_cache_count.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32]
+        L.vbto_frame_cache_576.argtypes = [ctypes.POINTER(VFile), ctypes.c_int, ctypes.c_void_p]
+        L.vbto_frame_cache_packed.argtypes = [ctypes.POINTER(VFile), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.vbto_dct_basis.restype = ctypes.c_float
+        L.vbto_dct_basis.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.vbto_sci_value_at.argtypes = [ctypes.POINTER(VFile)] + [ctypes.c_uint32] * 5 + [ctypes.POINTER(ctypes.c_float)]
+        L.vbto_sci_leaf_frame.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+        L.vbto_decode_render_leaf_frame.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32, ctypes.c_int,
+                                                    ctypes.POINTER(Compose), ctypes.c_void_p]
+        _O = L
+    return _O
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _R
+    if _R is None:
+        L = ctypes.CDLL(REF_SO)
+        L.vbtref_last_error.restype = ctypes.c_char_p
+        L.vbtref_half_to_float.restype = ctypes.c_float
+        L.vbtref_half_to_float.argtypes = [ctypes.c_uint16]
+        L.vbtref_float_to_half.restype = ctypes.c_uint16
+        L.vbtref_float_to_half.argtypes = [ctypes.c_float]
+        L.vbtref_decode_control_value.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint16, ctypes.c_uint8,
+                                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint16,
+                                                  ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.vbtref_parse_leaf.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+        L.vbtref_leaf_control_value.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_uint16, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.vbtref_shell_voxel_value.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint16,
+                                               ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.vbtref_load_file.restype = ctypes.c_void_p
+        L.vbtref_load_file.argtypes = [ctypes.c_char_p]
+        L.vbtref_free_file.argtypes = [ctypes.c_void_p]
+        L.vbtref_sci_value.argtypes = [ctypes.c_void_p] + [ctypes.c_uint32] * 4 + [ctypes.POINTER(ctypes.c_float)]
+        L.vbtref_sci_frame.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        _R = L
+    return _R
+
+
+def last_error() -> str:
+    return oracle().vbto_last_error().decode()
+
+
+def compose(sign_rule=0, background=0.0, shell_empty_scale=1.0, shell_empty_max=math.nan) -> Compose:
+    return Compose(sign_rule, background, shell_empty_scale, shell_empty_max)
+
+
+class OracleFile:
+    """A VBTPACK4 image validated by the oracle loader (keeps the bytes alive)."""
+
+    def __init__(self, data: bytes):
+        self.buf = np.frombuffer(bytearray(data), dtype=np.uint8)  # 4-byte aligned numpy allocation
+        self.vf = VFile()
+        rc = oracle().vbto_load_file_memory(self.buf.ctypes.data, len(data), ctypes.byref(self.vf))
+        if rc != 0:
+            raise ValueError(last_error())
+        h = self.vf.header
+        self.header = h
+        n_scale = h.coarseAcScaleCount
+        base = 52 + 4 * n_scale
+        self.offsets = self.buf[base: base + 4 * (h.leafCount + 1)].view(np.uint32)
+        self.payload = self.buf[base + 4 * (h.leafCount + 1):]
+
+    @property
+    def dims(self):
+        h = self.header
+        return h.width, h.height, h.depth, h.frames
+
+    def leaf_bytes(self, leaf: int) -> np.ndarray:
+        a, b = int(self.offsets[leaf]), int(self.offsets[leaf + 1])
+        return self.payload[4 * a: 4 * b]
+
+    def frame(self, t: int, comp: Compose | None = None, threads: int = 1) -> np.ndarray:
+        W, H, D, _ = self.dims
+        out = np.empty((D, H, W), np.float32)
+        comp = comp or compose()
+        rc = oracle().vbto_decode_frame_dense(ctypes.byref(self.vf), t, ctypes.byref(comp), out.ctypes.data, threads)
+        if rc != 0:
+            raise ValueError(last_error())
+        return out
+
+    def query(self, q: np.ndarray, comp: Compose | None = None, threads: int = 1, check=True) -> np.ndarray:
+        q = np.ascontiguousarray(q, dtype=np.uint32)
+        out = np.empty(len(q), np.float32)
+        comp = comp or compose()
+        rc = oracle().vbto_query_points(ctypes.byref(self.vf), q.ctypes.data, len(q), ctypes.byref(comp),
+                                        out.ctypes.data, threads)
+        if rc != 0 and check:
+            raise ValueError(last_error())
+        return out
+
+    def page_table(self):
+        L = self.header.leafCount
+        out = np.empty(L, np.uint32)
+        n = oracle().vbto_page_table(ctypes.byref(self.vf), out.ctypes.data)
+        return out, n
+
+    def cache_counts(self) -> np.ndarray:
+        return np.array([oracle().vbto_leaf_cache_count(ctypes.byref(self.vf), l)
+                         for l in range(self.header.leafCount)], dtype=np.uint32)
+
+    def cache_packed(self, t: int):
+        compact, n = self.page_table()
+        counts = self.cache_counts()
+        active = counts[compact != 0xFFFFFFFF] if self.header.profileType != 2 else counts
+        offs = np.zeros(len(active) + 1, np.uint64)
+        offs[1:] = np.cumsum(active, dtype=np.uint64)
+        out = np.zeros(int(offs[-1]), np.float32)
+        rc = oracle().vbto_frame_cache_packed(ctypes.byref(self.vf), t, offs.ctypes.data, out.ctypes.data)
+        if rc != 0:
+            raise ValueError(last_error())
+        return out, offs
+
+    def cache_576(self, t: int):
+        compact, n = self.page_table()
+        out = np.zeros(int(n) * 576, np.float32)
+        rc = oracle().vbto_frame_cache_576(ctypes.byref(self.vf), t, out.ctypes.data)
+        if rc != 0:
+            raise ValueError(last_error())
+        return out.reshape(int(n), 576)
+
+    def parse(self, leaf: int) -> LeafView:
+        b = self.leaf_bytes(leaf)
+        v = LeafView()
+        rc = oracle().vbto_parse_leaf(b.ctypes.data, len(b), 1, ctypes.byref(v))
+        if rc != 0:
+            raise ValueError(last_error())
+        return v
+
+    def modes(self) -> np.ndarray:
+        return (self.payload.view(np.uint32)[self.offsets[:-1]] & 3).astype(np.uint8)
