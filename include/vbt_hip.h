@@ -1,0 +1,286 @@
+/*
+ * vbt_hip.h -- C-ABI of libvbt_hip.so, the MI355X (gfx950) VBTPACK4 decoder.
+ *
+ * Plain C types only (pointers, sizes, PODs).  Device pointers are HIP device
+ * allocations (vbt_device_alloc, or any hipMalloc'd memory); work is ordered on
+ * the caller's hipStream_t passed as void* (NULL = the null stream).
+ * Every entry point returns VBT_OK (0) or an error code; the message of the
+ * last error on the calling thread is vbt_last_error().  Error texts of format
+ * failures are the reference's exception texts.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * the reference repository).  The C++ drop-in headers in include/vbt/ wrap
+ * these with the reference's own names and signatures.
+ */
+#ifndef VBT_HIP_H
+#define VBT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status -- */
+enum {
+    VBT_OK = 0,
+    VBT_ERR_INVALID = 1,   /* bad argument */
+    VBT_ERR_FORMAT = 2,    /* VBTPACK4 / leaf / stream format error (reference exception text) */
+    VBT_ERR_DEVICE = 3,    /* HIP runtime error */
+    VBT_ERR_NOMEM = 4,     /* device allocation failed */
+    VBT_ERR_BUDGET = 5,    /* server admission: request exceeds the HBM budget */
+    VBT_ERR_AGAIN = 6,     /* server: queue full / result not ready */
+    VBT_ERR_NO_DEVICE = 7, /* no HIP device present */
+};
+
+const char* vbt_last_error(void);
+/* Library version string and the offload target it was built for. */
+const char* vbt_version(void);
+
+/* ------------------------------------------------- format (host, pure) -- */
+/* Mirrors of src/render_temporal_payload.h and src/render_temporal_decode.h
+ * value types. */
+typedef struct {
+    uint8_t mode;        /* 0 EMPTY, 1 TEMPORAL_GRID4, 2 TEMPORAL_FINE_COMPACT (Mode2), 3 TEMPORAL_FINE6 */
+    uint8_t coarseCodec; /* 0 DP_KEYFRAME_FP16, 1 DP_KEYFRAME_FP32 */
+    uint8_t fineCodec;
+    uint8_t coarseResolution;
+    uint8_t fineResolution;
+} vbt_header_fields;         /* RenderTemporalPackedHeaderFields, render_temporal_payload.h:104-110 */
+
+typedef struct {
+    uint16_t coarseKeyframeCount;
+    uint16_t fineKeyframeCount;
+} vbt_leaf_prefix;           /* RenderTemporalPackedLeafPrefix, render_temporal_payload.h:115-118 */
+
+typedef struct {
+    uint32_t descriptorOffset, descriptorBytes, binIndexOffset, binIndexBytes, keyframeOffset, keyframeBytes,
+        totalBytes;
+} vbt_stream_layout;         /* RenderTemporalStreamLayout, render_temporal_payload.h:123-131 */
+
+typedef struct {
+    uint32_t headerBytes, prefixOffset, prefixBytes, coarseOffset;
+    vbt_stream_layout coarse;
+    uint32_t fineOffset;
+    vbt_stream_layout fine;
+    uint32_t totalBytes;
+} vbt_leaf_layout;           /* RenderTemporalPackedLeafLayout, render_temporal_payload.h:133-142 */
+
+typedef struct {
+    vbt_header_fields header;
+    vbt_leaf_prefix prefix;
+    vbt_leaf_layout layout;
+    uint16_t shellActiveVoxelCount;
+    uint32_t shellResidualOffset;
+    uint32_t shellResidualBytes;
+} vbt_leaf_view;             /* RenderTemporalPackedLeafView, render_temporal_payload.h:144-151 */
+
+typedef struct {
+    uint64_t shellMask[8];
+    uint16_t groupPrefix[8];
+} vbt_shell_section;         /* RenderTemporalShellOccupancySection, render_temporal_payload.h:170-173 */
+
+typedef struct {
+    uint16_t keyStart;
+    uint8_t keyCount;
+    uint8_t reserved;
+} vbt_control_descriptor;    /* RenderTemporalControlDescriptor, render_temporal_decode.h:11-15 */
+
+typedef struct {
+    uint16_t frame;
+    float value;
+} vbt_control_keyframe;      /* RenderTemporalControlKeyframe, render_temporal_decode.h:17-20 */
+
+/* RenderTemporalControlStreamData (render_temporal_decode.h:25-33) with
+ * caller-owned arrays. */
+typedef struct {
+    uint8_t codec;
+    uint16_t controlCount;
+    uint8_t timeBinCount;
+    uint8_t maxBinLocalKeys;
+    vbt_control_descriptor* descriptors; /* controlCount */
+    uint8_t* binIndex;                   /* controlCount * timeBinCount */
+    vbt_control_keyframe* keyframes;     /* keyframeCount */
+    uint32_t keyframeCount;
+} vbt_stream_data;
+
+uint16_t vbt_float_to_half_bits(float value);   /* renderTemporalFloatToHalfBits, render_temporal_decode.h:35 */
+float vbt_half_bits_to_float(uint16_t bits);    /* renderTemporalHalfBitsToFloat, render_temporal_decode.h:36 */
+uint32_t vbt_keyframe_bytes(uint32_t codec);    /* renderTemporalKeyframeBytes, render_temporal_payload.h:186 */
+uint32_t vbt_pack_header(const vbt_header_fields* fields);                /* render_temporal_payload.h:188 */
+void vbt_unpack_header(uint32_t packedHeader, vbt_header_fields* out);   /* render_temporal_payload.h:189 */
+/* makeRenderTemporalLeafPrefix, render_temporal_payload.h:191 (throws above u16 -> VBT_ERR_FORMAT) */
+int vbt_make_leaf_prefix(uint32_t mode, uint32_t coarseKeyframeCount, uint32_t fineKeyframeCount,
+                         vbt_leaf_prefix* out);
+/* validateRenderTemporalLeafPrefix, render_temporal_payload.h:195 (options reduced to timeBinCount) */
+int vbt_validate_leaf_prefix(const vbt_leaf_prefix* prefix, const vbt_header_fields* header, int timeBinCount);
+/* computeRenderTemporalLeafLayout, render_temporal_payload.h:199 */
+void vbt_compute_leaf_layout(const vbt_header_fields* header, const vbt_leaf_prefix* prefix, int alignTo4Bytes,
+                             vbt_leaf_layout* out);
+/* buildRenderTemporalPackedLeafBytes (5-arg), render_temporal_payload.h:203.  Writes at most outCap bytes;
+ * *outLen receives the full size (call with out = NULL to size). */
+int vbt_build_leaf_bytes(const vbt_header_fields* header, const vbt_leaf_prefix* prefix, const uint8_t* coarse,
+                         size_t coarseLen, const uint8_t* fine, size_t fineLen, int alignTo4Bytes, uint8_t* out,
+                         size_t outCap, size_t* outLen);
+/* buildRenderTemporalPackedLeafBytes (6-arg, Mode2), render_temporal_payload.h:211 */
+int vbt_build_shell_leaf_bytes(const vbt_header_fields* header, const vbt_leaf_prefix* prefix, const uint8_t* coarse,
+                               size_t coarseLen, const uint8_t* occupancy, size_t occupancyLen,
+                               const uint8_t* residual, size_t residualLen, int alignTo4Bytes, uint8_t* out,
+                               size_t outCap, size_t* outLen);
+void vbt_pack_shell_section(const vbt_shell_section* s, uint8_t out[80]);        /* render_temporal_payload.h:218 */
+void vbt_unpack_shell_section(const uint8_t* data, uint64_t size, vbt_shell_section* out); /* :219 */
+uint16_t vbt_shell_active_count(const vbt_shell_section* s);                      /* :220 */
+/* parseRenderTemporalPackedLeaf, render_temporal_payload.h:222 */
+int vbt_parse_leaf(const uint8_t* leafBase, size_t leafBytes, int alignTo4Bytes, vbt_leaf_view* out);
+/* validateRenderTemporalControlStreamData, render_temporal_decode.h:38 -> 1 valid, 0 invalid */
+int vbt_validate_control_stream(const vbt_stream_data* stream);
+/* packRenderTemporalControlStream, render_temporal_decode.h:39 */
+int vbt_pack_control_stream(const vbt_stream_data* stream, int alignTo4Bytes, uint8_t* out, size_t outCap,
+                            size_t* outLen);
+/* unpackRenderTemporalControlStream, render_temporal_decode.h:42; out's arrays are caller-allocated
+ * (controlCount descriptors, controlCount*timeBinCount bins, keyframeCount keyframes). */
+int vbt_unpack_control_stream(const uint8_t* streamBase, uint32_t codec, uint16_t controlCount, uint8_t timeBinCount,
+                              uint8_t maxBinLocalKeys, uint32_t keyframeCount, uint32_t streamBytes, int alignTo4Bytes,
+                              vbt_stream_data* out);
+
+/* --------------------------------------------- per-value decode (GPU) -- */
+/* Batched forms of the reference's per-value decoders: request i decodes
+ * (controlIndex[i], frameIndex[i]) and writes out[i]; status[i] = 0 or a
+ * VBT_LEAF_* code (the value is then NaN).  All pointers are device memory. */
+/* decodeRenderTemporalControlValue, render_temporal_decode.h:51 */
+int vbt_decode_control_values(const uint8_t* d_stream, uint32_t codec, uint16_t controlCount, uint8_t timeBinCount,
+                              uint8_t maxBinLocalKeys, uint32_t keyframeCount, uint32_t streamBytes, int alignTo4Bytes,
+                              int totalFrames, const uint16_t* d_controlIndex, const int32_t* d_frameIndex, uint32_t n,
+                              float* d_out, int32_t* d_status, void* stream);
+/* decodeRenderTemporalLeafControlValue, render_temporal_decode.h:63 (leaf parsed on the device) */
+int vbt_decode_leaf_control_values(const uint8_t* d_leaf, uint32_t leafBytes, int useFineStream, int alignTo4Bytes,
+                                   int totalFrames, const uint16_t* d_controlIndex, const int32_t* d_frameIndex,
+                                   uint32_t n, float* d_out, int32_t* d_status, void* stream);
+/* decodeRenderTemporalLeafShellVoxelValue, render_temporal_decode.h:76 */
+int vbt_decode_shell_voxel_values(const uint8_t* d_leaf, uint32_t leafBytes, int alignTo4Bytes, int totalFrames,
+                                  const uint16_t* d_voxelIndex, const int32_t* d_frameIndex, uint32_t n, float* d_out,
+                                  int32_t* d_status, void* stream);
+
+/* Leaf status codes written to d_status / reported per leaf. */
+enum {
+    VBT_LEAF_OK = 0,
+    VBT_LEAF_ERR_TOO_SMALL = 1,
+    VBT_LEAF_ERR_PREFIX_MISSING = 2,
+    VBT_LEAF_ERR_SHORT = 3,
+    VBT_LEAF_ERR_OCCUPANCY_OOB = 4,
+    VBT_LEAF_ERR_GROUP_PREFIX = 5,
+    VBT_LEAF_ERR_NO_ACTIVE = 6,
+    VBT_LEAF_ERR_RESIDUAL_OOB = 7,
+    VBT_LEAF_ERR_TRAILING = 8,
+    VBT_LEAF_ERR_KEY_RANGE = 9,
+    VBT_LEAF_ERR_STREAM_OOB = 10,
+    VBT_LEAF_ERR_SCI_RANGE = 11,
+    VBT_QUERY_ERR_OUTSIDE = 32,      /* point query voxel outside the volume */
+    VBT_QUERY_ERR_FRAME = 33,        /* scientific query frame >= frames */
+    VBT_ERR_CONTROL_INDEX = 34,      /* control index >= controlCount */
+};
+/* The reference's exception text for a status code. */
+const char* vbt_status_message(int32_t status);
+
+/* ------------------------------------------------- assets (container) -- */
+typedef struct {
+    uint32_t version, width, height, depth, frames, leafSize, coarseResolution, maxCoarseKeep, leafCount,
+        profileType, coarseAcScaleCount;
+} vbt_file_header;           /* VbtFileHeader, render/src/vbt_file.h:10-23 */
+
+/* loadVbtFile's validation (render/src/vbt_file.cpp:174-275) over an in-memory
+ * image: same checks, same messages. */
+int vbt_validate_file(const void* bytes, uint64_t size, vbt_file_header* out);
+/* leafIndexForVoxel, render/src/vbt_file.h:63 */
+uint32_t vbt_leaf_index_for_voxel(const vbt_file_header* h, uint32_t x, uint32_t y, uint32_t z);
+
+typedef struct vbt_asset_s* vbt_asset;
+
+/* Validate (as loadVbtFile), upload the offset table, payload pool and AC
+ * scales to HBM, then build the page table on the device: the compact index
+ * of every leaf (rank among active leaves in leaf order; 0xffffffff when
+ * EMPTY -- the deterministic form of vbt_active_index.comp), the per-leaf
+ * decode plan, the packed-cache control offsets and the per-leaf status.
+ * Runs on the given stream and synchronises it once at the end. */
+int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_asset* out);
+int vbt_asset_open_file(const char* path, void* stream, vbt_asset* out);
+void vbt_asset_close(vbt_asset a);
+
+typedef struct {
+    vbt_file_header header;
+    uint32_t activeLeaves;       /* A */
+    uint32_t badLeaves;          /* leaves whose status != VBT_LEAF_OK */
+    uint64_t controlsPerFrame;   /* packed cache floats per frame (meaningful controls only) */
+    uint64_t payloadBytes;       /* compressed payload resident in HBM */
+    uint64_t deviceBytes;        /* all device memory owned by the asset */
+    uint64_t mode_counts[4];     /* leaves per header mode (render) / scientific mode */
+    uint64_t fast_leaves;        /* active leaves eligible for the multi-frame sweep */
+} vbt_asset_info;
+int vbt_asset_info_get(vbt_asset a, vbt_asset_info* out);
+
+/* Device-side page table copies (device pointers; sizes from the info). */
+int vbt_asset_page_table(vbt_asset a, uint32_t* d_compact /* leafCount */, void* stream);
+int vbt_asset_cache_offsets(vbt_asset a, uint64_t* d_offsets /* activeLeaves + 1 */, void* stream);
+int vbt_asset_leaf_status(vbt_asset a, uint32_t* d_status /* leafCount */, void* stream);
+
+/* ------------------------------------------------------ frame caches -- */
+enum {
+    VBT_CACHE_PACKED = 0,    /* out[(t-t0)*controlsPerFrame + offset[a] + slot], meaningful controls only */
+    VBT_CACHE_STRIDE576 = 1, /* out[(t-t0)*A*576 + a*576 + slot], vbt_cache.comp slot layout, unused slots 0 */
+};
+/* Decode frames [t0, t0+nt) of every active leaf into the compact active-leaf
+ * cache.  Render leaves: coarse controls then Fine6 or Mode2 shell controls
+ * (vbt_cache.comp:193-236 slot order, CPU arithmetic).  Scientific leaves: the
+ * leaf's decoded voxels (cropped, x fastest).  Returns VBT_ERR_FORMAT when any
+ * active leaf is poisoned (its slots are NaN, the others are valid). */
+int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out, void* stream);
+
+/* --------------------------------------------------- voxels & queries -- */
+enum {
+    VBT_SIGN_NONE = 0,        /* render_temporal_vbt_to_vdb default (optional shellEmptyScale) */
+    VBT_SIGN_CLAMP_ZERO = 1,  /* render_smoke_vbt_bench_main.cpp:374-378: Mode2 !occupied && v<0 -> 0 */
+    VBT_SIGN_BACKGROUND = 2,  /* vbt_sample.comp:410-412: Mode2 !occupied && v<0 -> background */
+};
+typedef struct {
+    int signRule;
+    float background;
+    float shellEmptyScale; /* 1 disables */
+    float shellEmptyMax;   /* NaN disables */
+} vbt_compose;
+
+/* decodeFrameToDense (render/src/render_smoke_vbt_bench_main.cpp:389-455) /
+ * reconstructScientificFrameCpu (render/src/scientific_decode.h:12): frame t,
+ * row-major (z, y, x) floats. */
+int vbt_decode_frame_dense(vbt_asset a, int t, const vbt_compose* c, float* d_out, void* stream);
+
+/* Point queries (Query4D x,y,z,t as 4 x u32, render/src/query_patterns.h:11-17;
+ * the leaf is leafIndexForVoxel(x,y,z)): value = the whole-frame decode at
+ * that voxel.  d_status may be NULL.  decodeScientificValueAtCpu,
+ * render/src/scientific_decode.h:11, and the render point sampler. */
+int vbt_query_points(vbt_asset a, const uint32_t* d_queries, uint32_t n, const vbt_compose* c, float* d_out,
+                     int32_t* d_status, void* stream);
+/* The plain GPU decode used as the comparison point: one thread per query,
+ * one launch per request, no grouping. */
+int vbt_query_points_plain(vbt_asset a, const uint32_t* d_queries, uint32_t n, const vbt_compose* c, float* d_out,
+                           int32_t* d_status, void* stream);
+
+/* ------------------------------------------- device / stream helpers -- */
+int vbt_device_count(int* n);
+int vbt_device_alloc(void** d_ptr, size_t bytes);
+int vbt_device_free(void* d_ptr);
+int vbt_memcpy_h2d(void* d_dst, const void* src, size_t bytes, void* stream);
+int vbt_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream);
+int vbt_memset_d(void* d_dst, int value, size_t bytes, void* stream);
+int vbt_stream_create(void** stream);
+int vbt_stream_destroy(void* stream);
+int vbt_stream_synchronize(void* stream);
+int vbt_event_create(void** ev);
+int vbt_event_destroy(void* ev);
+int vbt_event_record(void* ev, void* stream);
+int vbt_event_elapsed_ms(void* start, void* stop, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

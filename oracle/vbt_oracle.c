@@ -533,6 +533,7 @@ static void locate(float p, int extent, int res, int* i0o, int* i1o, float* wo)
 /* sampleControlGrid, render_temporal_vbt_to_vdb.cpp:136-167 */
 static float sample_grid(const float* g, int res, int w, int h, int d, float fx, float fy, float fz)
 {
+    if (res < 1) return 0.0f; /* empty grid: the reference indexes an empty vector (undefined); defined as 0 */
     int x0, x1, y0, y1, z0, z1;
     float tx, ty, tz;
     locate(fx, w, res, &x0, &x1, &tx);
@@ -1003,6 +1004,7 @@ static int point_grid(const uint8_t* base, const vbto_leaf_view* v, int useFine,
 {
     int x0, x1, y0, y1, z0, z1;
     float tx, ty, tz;
+    if (res < 1) { *out = 0.0f; return VBTO_OK; }
     locate((float)lx, lw, res, &x0, &x1, &tx);
     locate((float)ly, lh, res, &y0, &y1, &ty);
     locate((float)lz, ld, res, &z0, &z1, &tz);

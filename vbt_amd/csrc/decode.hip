@@ -1,0 +1,1347 @@
+// decode.hip -- CDNA4 (gfx950) kernels of the VBTPACK4 random-access decode
+// path and the asset / C-ABI layer around them.
+//
+// Data resident in HBM per asset (all uploaded once at open):
+//   offsets  u32[L+1]   word offsets of the leaves (render/src/vbt_file.cpp layout)
+//   payload  u32[P]     the leaf payload pool, byte-addressed by the kernels
+//   scales   f32[S]     scientific AC scales
+//   compact  u32[L]     page table: rank among active leaves (0xffffffff = EMPTY)
+//   status   u32[L]     per-leaf status (LEAF_* | FLAG_SWEEP)
+//   leaves   DevLeaf[A] per active leaf decode plan (compact order)
+//   ctrl     u64[A+1]   packed-cache control offsets (prefix sum of C(l))
+//   basis    f32[F*64]  scientific DCT basis table (dctBasis values, host-computed)
+//
+// Kernels (one launch each, stream-ordered):
+//   k_scan_a / k_scan_b / k_scan_c   deterministic page table: wave64 ballot +
+//                                    popcount ranks, LDS block scan, block-base scan
+//   k_validate                       descriptor ranges + keyframe monotonicity per leaf
+//   k_cache_render                   multi-frame control cache (sweep or exact bracket)
+//   k_cache_sci                      scientific leaf voxels per frame (DCT in fp64)
+//   k_dense_render / k_dense_sci     whole-frame voxel composition
+//   k_query / k_query_plain          point queries (8 lanes per query / 1 thread per query)
+//   k_control_values / ...           batched forms of the reference per-value decoders
+#include "vbt_format.h"
+#include "vbt_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+using namespace vbtk;
+
+// ------------------------------------------------------------ error state --
+static thread_local std::string g_err;
+
+int vbt_fail(int code, const char* msg)
+{
+    g_err = msg;
+    return code;
+}
+int vbt_fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return vbt_fail(e_ == hipErrorOutOfMemory ? VBT_ERR_NOMEM : VBT_ERR_DEVICE,                 \
+                            std::string(#expr " failed: ") + hipGetErrorString(e_));                    \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t FLAG_SWEEP = 1u << 8;  // every control's keyframe frames strictly increase
+constexpr uint32_t STATUS_MASK = 0xFFu;
+constexpr uint32_t NO_LEAF = 0xffffffffu;
+
+struct DevLeaf {
+    uint32_t leaf, word, bytes, status;
+    uint32_t ncache, cc;       // cache slots; coarse slots
+    uint32_t lw, lh, ld;       // cropped extent
+    uint32_t sciHeader;        // scientific header word
+    LeafPlan plan;
+};
+
+__host__ __device__ inline void leaf_coords(uint32_t leaf, uint32_t W, uint32_t H, uint32_t D, uint32_t& bx,
+                                            uint32_t& by, uint32_t& bz, uint32_t& lw, uint32_t& lh, uint32_t& ld)
+{
+    const uint32_t lcx = (W + 7u) / 8u, lcy = (H + 7u) / 8u;
+    bx = leaf % lcx;
+    by = (leaf / lcx) % lcy;
+    bz = leaf / (lcx * lcy);
+    lw = W - bx * 8u < 8u ? W - bx * 8u : 8u;
+    lh = H - by * 8u < 8u ? H - by * 8u : 8u;
+    ld = D - bz * 8u < 8u ? D - bz * 8u : 8u;
+}
+
+struct AssetDev {
+    const uint32_t* offsets;
+    const uint8_t* payload;
+    const float* scales;
+    const uint32_t* compact;
+    const DevLeaf* leaves;
+    const uint64_t* ctrl;
+    const float* basis;
+    uint32_t W, H, D, F, L, A, profile, scaleCount;
+    uint64_t payloadBytes;
+};
+
+// ------------------------------------------------------------ wave helpers --
+__device__ inline uint32_t wave_prefix_u32(uint32_t v, uint32_t lane, uint32_t& total)
+{
+    // inclusive scan across the 64-lane wavefront
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v; // exclusive
+}
+
+// per-leaf active flag and cache-slot count (0 for EMPTY and poisoned leaves)
+__device__ inline void leaf_count(const AssetDev& a, uint32_t l, uint32_t& active, uint32_t& n, uint32_t& status,
+                                  LeafPlan& pl)
+{
+    active = 0;
+    n = 0;
+    status = LEAF_OK;
+    const uint32_t word = a.offsets[l];
+    const uint32_t bytes = (a.offsets[l + 1] - word) * 4u;
+    const Bytes b{a.payload + (size_t)word * 4u};
+    if (a.profile == 2u) {
+        uint32_t bx, by, bz, lw, lh, ld;
+        leaf_coords(l, a.W, a.H, a.D, bx, by, bz, lw, lh, ld);
+        active = 1;
+        n = lw * lh * ld;
+        pl.mode = b.u32(0) & 3u;
+        return;
+    }
+    status = plan_leaf(b, bytes, pl);
+    active = pl.mode != MODE_EMPTY ? 1u : 0u;
+    if (active && status == LEAF_OK) {
+        n = control_count(pl.coarseRes);
+        if (pl.mode == MODE_FINE6) n += control_count(pl.fineRes);
+        if (pl.mode == MODE_SHELL) n += pl.shell.count;
+    }
+}
+
+struct Pair {
+    uint32_t active;
+    uint64_t n;
+};
+
+// pass A: per-block totals of (active leaves, cache slots)
+__global__ void __launch_bounds__(256) k_scan_a(AssetDev a, Pair* blockSums)
+{
+    __shared__ uint32_t sA[4];
+    __shared__ uint32_t sN[4];
+    const uint32_t l = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t act = 0, n = 0, st;
+    LeafPlan pl;
+    if (l < a.L) leaf_count(a, l, act, n, st, pl);
+    const uint64_t m = __ballot(act);
+    uint32_t tot;
+    wave_prefix_u32(n, lane, tot);
+    if (lane == 0) {
+        sA[wave] = (uint32_t)__popcll(m);
+        sN[wave] = tot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Pair p{0, 0};
+        for (int w = 0; w < 4; ++w) {
+            p.active += sA[w];
+            p.n += sN[w];
+        }
+        blockSums[blockIdx.x] = p;
+    }
+}
+
+// pass B: exclusive scan of block totals by one workgroup (deterministic)
+__global__ void __launch_bounds__(1024) k_scan_b(Pair* blockSums, uint32_t nb, Pair* total)
+{
+    __shared__ uint32_t sA[16];
+    __shared__ uint64_t sN[16];
+    __shared__ Pair carry;
+    if (threadIdx.x == 0) carry = Pair{0, 0};
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < nb; base += 1024u) {
+        const uint32_t i = base + threadIdx.x;
+        Pair v = i < nb ? blockSums[i] : Pair{0, 0};
+        // wave inclusive scans (64-bit for slot counts)
+        uint32_t xa = v.active;
+        uint64_t xn = v.n;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t ya = __shfl_up(xa, d, 64);
+            const uint64_t yn = __shfl_up(xn, d, 64);
+            if (lane >= (uint32_t)d) {
+                xa += ya;
+                xn += yn;
+            }
+        }
+        if (lane == 63u) {
+            sA[wave] = xa;
+            sN[wave] = xn;
+        }
+        __syncthreads();
+        uint32_t wa = 0;
+        uint64_t wn = 0;
+        for (uint32_t w = 0; w < wave; ++w) {
+            wa += sA[w];
+            wn += sN[w];
+        }
+        const Pair c = carry;
+        if (i < nb) blockSums[i] = Pair{c.active + wa + xa - v.active, c.n + wn + xn - v.n};
+        __syncthreads();
+        if (threadIdx.x == 1023u) carry = Pair{c.active + wa + xa, c.n + wn + xn};
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+// pass C: ranks, page table, per-leaf plans
+__global__ void __launch_bounds__(256) k_scan_c(AssetDev a, const Pair* blockBase, uint32_t* compact, uint32_t* status,
+                                                DevLeaf* leaves, uint64_t* ctrl)
+{
+    __shared__ uint32_t sA[4];
+    __shared__ uint32_t sN[4];
+    const uint32_t l = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t act = 0, n = 0, st = LEAF_OK;
+    LeafPlan pl;
+    if (l < a.L) leaf_count(a, l, act, n, st, pl);
+    const uint64_t m = __ballot(act);
+    const uint32_t rankInWave = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    uint32_t tot;
+    const uint32_t nEx = wave_prefix_u32(n, lane, tot);
+    if (lane == 0) {
+        sA[wave] = (uint32_t)__popcll(m);
+        sN[wave] = tot;
+    }
+    __syncthreads();
+    uint32_t wa = 0, wn = 0;
+    for (uint32_t w = 0; w < wave; ++w) {
+        wa += sA[w];
+        wn += sN[w];
+    }
+    if (l >= a.L) return;
+    const Pair base = blockBase[blockIdx.x];
+    status[l] = st;
+    if (!act) {
+        compact[l] = NO_LEAF;
+        return;
+    }
+    const uint32_t c = base.active + wa + rankInWave;
+    compact[l] = c;
+    ctrl[c] = base.n + wn + nEx;
+    DevLeaf d;
+    d.leaf = l;
+    d.word = a.offsets[l];
+    d.bytes = (a.offsets[l + 1] - d.word) * 4u;
+    d.status = st;
+    d.ncache = n;
+    d.cc = a.profile == 2u ? 0u : control_count(pl.coarseRes);
+    uint32_t bx, by, bz;
+    leaf_coords(l, a.W, a.H, a.D, bx, by, bz, d.lw, d.lh, d.ld);
+    d.sciHeader = a.profile == 2u ? a.payload[(size_t)d.word * 4u] | ((uint32_t)a.payload[(size_t)d.word * 4u + 1] << 8) |
+                                        ((uint32_t)a.payload[(size_t)d.word * 4u + 2] << 16) |
+                                        ((uint32_t)a.payload[(size_t)d.word * 4u + 3] << 24)
+                                  : 0u;
+    d.plan = pl;
+    leaves[c] = d;
+}
+
+// maximum payload byte (exclusive) a scientific leaf decode can read
+__device__ inline uint64_t sci_read_end(const AssetDev& a, const DevLeaf& d)
+{
+    const SciHeader h = sci_header(d.sciHeader);
+    const uint64_t base = (uint64_t)d.word * 4u;
+    uint64_t end = base + 4u + 64u * (h.keep + 1u);
+    if (h.mode == 1u) {
+        const uint32_t cap = h.tier == 0 ? 64u : h.tier == 1 ? 256u : h.tier == 2 ? 768u : 1024u;
+        const uint64_t sparse = base + 4u + 64u * (h.keep + 1u);
+        const uint64_t coords = sparse + 4u + 64u + 32u;
+        uint64_t e = coords;
+        if (h.count > 0) {
+            e = coords + 2u * (uint64_t)(h.count > cap ? h.count : cap);
+            const uint64_t nib = coords + 2u * cap + (h.count + 1u) / 2u;
+            if (nib > e) e = nib;
+        }
+        if (e > end) end = e;
+    } else if (h.mode >= 2u) {
+        const uint32_t R = h.mode == 2u ? 3u : 4u;
+        const uint64_t vals = (uint64_t)R * R * R;
+        const uint64_t blockBytes = (vals * h.bits + 7u) / 8u;
+        const uint64_t scales = base + 4u + 64u * (h.keep + 1u);
+        const uint64_t nb = h.subtype == 1u ? h.tk : a.F;
+        const uint64_t e = scales + 2u * nb + nb * blockBytes + 1u; // BFP reads may touch one byte past a block
+        if (h.subtype != 1u || (h.tk > 0 && h.bits > 0))
+            if (e > end) end = e;
+    }
+    return end;
+}
+
+// validation: one workgroup per active leaf
+__global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, uint32_t* status)
+{
+    __shared__ uint32_t sErr, sMono;
+    const uint32_t c = blockIdx.x;
+    DevLeaf& d = leaves[c];
+    if (threadIdx.x == 0) {
+        sErr = d.status & STATUS_MASK;
+        sMono = 1u;
+    }
+    __syncthreads();
+    if (a.profile == 2u) {
+        if (threadIdx.x == 0) {
+            const SciHeader h = sci_header(d.sciHeader);
+            uint32_t st = LEAF_OK;
+            if (h.keep > 1u && h.keep - 1u > a.scaleCount) st = LEAF_ERR_SCI_RANGE;
+            if (sci_read_end(a, d) > a.payloadBytes) st = LEAF_ERR_SCI_RANGE;
+            d.status = st;
+            status[d.leaf] = st;
+        }
+        return;
+    }
+    if (sErr == LEAF_OK) {
+        const Bytes b{a.payload + (size_t)d.word * 4u};
+        const StreamRef* streams[3] = {&d.plan.coarse, d.plan.mode == MODE_FINE6 || d.plan.hasFine ? &d.plan.fine : nullptr,
+                                       d.plan.mode == MODE_SHELL ? &d.plan.shell : nullptr};
+        for (int s = 0; s < 3; ++s) {
+            if (!streams[s]) continue;
+            const StreamRef sr = *streams[s];
+            for (uint32_t i = threadIdx.x; i < sr.count; i += blockDim.x) {
+                const uint32_t ks = b.u16(sr.base + 4u * i), kc = b.u8(sr.base + 4u * i + 2u);
+                if (kc == 0u) continue;
+                if (ks + kc > sr.keys) {
+                    atomicMax(&sErr, (uint32_t)LEAF_ERR_KEY_RANGE);
+                    continue;
+                }
+                int prev = key_frame(b, sr.keyBase, sr.codec, ks);
+                for (uint32_t j = 1; j < kc; ++j) {
+                    const int f = key_frame(b, sr.keyBase, sr.codec, ks + j);
+                    if (f <= prev) {
+                        sMono = 0u;
+                        break;
+                    }
+                    prev = f;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t st = sErr | (sErr == LEAF_OK && sMono ? FLAG_SWEEP : 0u);
+        d.status = st;
+        status[d.leaf] = st;
+    }
+}
+
+__device__ inline uint32_t find_leaf(const uint64_t* ctrl, uint32_t A, uint64_t g)
+{
+    uint32_t lo = 0, hi = A; // ctrl[lo] <= g < ctrl[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ctrl[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------- cache: render --
+// One thread per (active leaf, cache slot); the thread produces that control
+// for frames t0..t0+nt-1.  Leaves whose every control has strictly increasing
+// keyframe frames use a forward sweep over the keyframes (each frame's bracket
+// is the adjacent key pair around it, which is exactly the bracket the
+// reference's hinted binary search returns for increasing keys); other leaves
+// run the reference bracket search per frame.
+template <int LAYOUT>
+__global__ void __launch_bounds__(256) k_cache_render(AssetDev a, int t0, int nt, float* out, uint64_t total,
+                                                      uint32_t* errFlag)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t c = find_leaf(a.ctrl, a.A, g);
+    const DevLeaf& d = a.leaves[c];
+    const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
+    const uint32_t status = d.status;
+    uint64_t stride;
+    uint64_t dst;
+    if (LAYOUT == VBT_CACHE_PACKED) {
+        stride = total;
+        dst = g;
+    } else {
+        const uint32_t s576 = slot < d.cc ? slot : 64u + (slot - d.cc);
+        if ((slot < d.cc && slot >= 64u) || s576 >= 576u) return;
+        stride = (uint64_t)a.A * 576u;
+        dst = (uint64_t)c * 576u + s576;
+    }
+    out += dst;
+    if ((status & STATUS_MASK) != LEAF_OK) {
+        for (int i = 0; i < nt; ++i) out[(uint64_t)i * stride] = __builtin_nanf("");
+        atomicOr(errFlag, 1u);
+        return;
+    }
+    const bool isCoarse = slot < d.cc;
+    const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
+    const uint32_t ci = isCoarse ? slot : slot - d.cc;
+    const Bytes b{a.payload + (size_t)d.word * 4u};
+    const int F = (int)a.F;
+    const uint32_t dsc = s.base + ci * 4u;
+    const uint32_t keyStart = b.u16(dsc), keyCount = b.u8(dsc + 2u);
+    if (keyCount == 0u) {
+        for (int i = 0; i < nt; ++i) out[(uint64_t)i * stride] = 0.0f;
+        return;
+    }
+    if (!(status & FLAG_SWEEP)) {
+        const uint32_t binBase = s.base + s.count * 4u + ci * s.bins;
+        for (int i = 0; i < nt; ++i)
+            out[(uint64_t)i * stride] =
+                decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins, F, t0 + i);
+        return;
+    }
+    // sweep: k = last key with frame <= current frame (or 0)
+    const uint32_t last = keyCount - 1u;
+    uint32_t k = 0;
+    int fk = key_frame(b, s.keyBase, s.codec, keyStart);
+    float vk = key_value(b, s.keyBase, s.codec, keyStart);
+    int fn = last > 0u ? key_frame(b, s.keyBase, s.codec, keyStart + 1u) : 0;
+    float vn = last > 0u ? key_value(b, s.keyBase, s.codec, keyStart + 1u) : 0.0f;
+    for (int i = 0; i < nt; ++i) {
+        const int frame = clamp_frame(t0 + i, F);
+        while (k < last && fn <= frame) {
+            ++k;
+            fk = fn;
+            vk = vn;
+            if (k < last) {
+                fn = key_frame(b, s.keyBase, s.codec, keyStart + k + 1u);
+                vn = key_value(b, s.keyBase, s.codec, keyStart + k + 1u);
+            }
+        }
+        float v;
+        if (fk >= frame || k == last) {
+            v = vk;
+        } else {
+            const float alpha = (float)(frame - fk) / (float)(fn - fk);
+            v = vk + alpha * (vn - vk);
+        }
+        out[(uint64_t)i * stride] = v;
+    }
+}
+
+// ------------------------------------------------------ scientific decode --
+__device__ inline float sci_control(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t ci, int t)
+{
+    const uint32_t cb = 4u + ci * (h.keep + 1u);
+    const float* basis = a.basis + (size_t)t * 64u;
+    double sum = (double)half_to_float(b.u16(cb)) * (double)basis[0];
+    for (uint32_t k = 1; k < h.keep; ++k) {
+        const float coeff = (float)(int8_t)(uint8_t)b.u8(cb + 2u + (k - 1u)) * a.scales[k - 1u];
+        const double term = (double)coeff * (double)basis[k];
+        sum += term;
+    }
+    return (float)sum;
+}
+
+__device__ inline uint32_t bfp_code(const Bytes& b, uint32_t blockBase, uint32_t ci, uint32_t bits)
+{
+    const uint32_t bit = ci * bits;
+    const uint32_t byteIndex = blockBase + bit / 8u;
+    const uint32_t shift = bit % 8u;
+    uint32_t packed = b.u8(byteIndex);
+    if (shift + bits > 8u) packed |= b.u8(byteIndex + 1u) << 8;
+    return (packed >> shift) & ((1u << bits) - 1u);
+}
+
+__device__ inline float sci_dense_value(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t lx, uint32_t ly,
+                                        uint32_t lz, uint32_t t)
+{
+    const int R = h.mode == 2u ? 3 : 4;
+    const uint32_t bits = h.bits;
+    const uint32_t vals = (uint32_t)(R * R * R);
+    const uint32_t blockBytes = (vals * bits + 7u) / 8u;
+    const uint32_t scalesBase = 4u + 64u * (h.keep + 1u);
+    int x0, y0, z0;
+    float tx, ty, tz;
+    sci_dense_coord(lx, R, x0, tx);
+    sci_dense_coord(ly, R, y0, ty);
+    sci_dense_coord(lz, R, z0, tz);
+    if (h.subtype == 1u) {
+        if (h.tk == 0u || bits == 0u) return 0.0f;
+        const uint32_t dataBase = scalesBase + h.tk * 2u;
+        float residual = 0.0f;
+        for (uint32_t k = 0; k < h.tk; ++k) {
+            const float scale = half_to_float(b.u16(scalesBase + k * 2u));
+            const uint32_t bb = dataBase + k * blockBytes;
+            float s[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
+                s[i] = decode_bfp(bfp_code(b, bb, (uint32_t)((cz * R + cy) * R + cx), bits), scale, (int)bits);
+            }
+            residual += trilinear(s, tx, ty, tz) * a.basis[(size_t)t * 64u + k];
+        }
+        return residual;
+    }
+    const uint32_t dataBase = scalesBase + a.F * 2u;
+    const float scale = half_to_float(b.u16(scalesBase + t * 2u));
+    const uint32_t frameBase = dataBase + t * blockBytes;
+    float s[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
+        s[i] = decode_bfp(bfp_code(b, frameBase, (uint32_t)((cz * R + cy) * R + cx), bits), scale, (int)bits);
+    }
+    return trilinear(s, tx, ty, tz);
+}
+
+__device__ inline float sci_sparse(const Bytes& b, const SciHeader& h, uint32_t idx, uint32_t t)
+{
+    if (h.count == 0u) return 0.0f;
+    const uint32_t cap = h.tier == 0 ? 64u : h.tier == 1 ? 256u : h.tier == 2 ? 768u : 1024u;
+    const uint32_t bin = t / h.fpb < 15u ? t / h.fpb : 15u;
+    const uint32_t localT = (t - bin * h.fpb) & 0xFFu;
+    const uint32_t sparseBase = 4u + 64u * (h.keep + 1u);
+    const float eventScale = bits_to_float(b.u32(sparseBase));
+    if (!(eventScale > 0.0f)) return 0.0f;
+    const uint32_t maskWord = b.u32(sparseBase + 4u + (idx >> 5) * 4u);
+    if ((maskWord & (1u << (idx & 31u))) == 0u) return 0.0f;
+    const uint32_t binsBase = sparseBase + 68u;
+    const uint32_t start = b.u16(binsBase + bin * 2u);
+    const uint32_t end = bin + 1u < 16u ? b.u16(binsBase + (bin + 1u) * 2u) : (h.count & 0xFFFFu);
+    if (start >= end) return 0.0f;
+    const uint32_t coordsBase = binsBase + 32u;
+    const uint32_t residualBase = coordsBase + cap * 2u;
+    const uint32_t target = ((localT << 9) | (idx & 0x1FFu)) & 0xFFFFu;
+    int left = (int)start, right = (int)end - 1;
+    while (left <= right) {
+        const int mid = (left + right) >> 1;
+        const uint32_t key = b.u16(coordsBase + (uint32_t)mid * 2u);
+        if (key == target) {
+            const uint32_t byte = b.u8(residualBase + (uint32_t)(mid >> 1));
+            const uint32_t nib = (mid & 1) == 0 ? (byte & 0xFu) : ((byte >> 4) & 0xFu);
+            int q = (int)nib;
+            if (q & 8) q -= 16;
+            return (float)q * eventScale;
+        }
+        if (key < target) left = mid + 1;
+        else right = mid - 1;
+    }
+    return 0.0f;
+}
+
+__device__ inline float sci_residual(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t lx, uint32_t ly,
+                                     uint32_t lz, uint32_t t)
+{
+    if (h.mode == 0u) return 0.0f;
+    if (h.mode == 1u) return sci_sparse(b, h, (lz * 8u + ly) * 8u + lx, t);
+    return sci_dense_value(a, b, h, lx, ly, lz, t);
+}
+
+__device__ inline float sci_coarse_sample(const float* ctrl, uint32_t lx, uint32_t ly, uint32_t lz)
+{
+    int x0, y0, z0;
+    float tx, ty, tz;
+    sci_coarse_coord(lx, x0, tx);
+    sci_coarse_coord(ly, y0, ty);
+    sci_coarse_coord(lz, z0, tz);
+    float s[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
+        s[i] = ctrl[(cz * 4 + cy) * 4 + cx];
+    }
+    return trilinear(s, tx, ty, tz);
+}
+
+// scientific leaf-frames: one workgroup per (active leaf, frame), 512 lanes = voxels
+template <bool DENSE_OUT>
+__global__ void __launch_bounds__(512) k_sci_leaf(AssetDev a, int t0, int nt, float* out, uint64_t total,
+                                                  uint32_t* errFlag)
+{
+    __shared__ float ctrl[64];
+    const uint32_t c = blockIdx.x;
+    const int t = t0 + (int)blockIdx.y;
+    const DevLeaf& d = a.leaves[c];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lx = tid & 7u, ly = (tid >> 3) & 7u, lz = tid >> 6;
+    const bool inside = lx < d.lw && ly < d.lh && lz < d.ld;
+    uint64_t dst;
+    if (DENSE_OUT) {
+        uint32_t bx, by, bz, lw, lh, ld;
+        leaf_coords(d.leaf, a.W, a.H, a.D, bx, by, bz, lw, lh, ld);
+        dst = ((uint64_t)(bz * 8u + lz) * a.H + (by * 8u + ly)) * a.W + bx * 8u + lx;
+    } else {
+        dst = (uint64_t)blockIdx.y * total + a.ctrl[c] + (lz * d.lh + ly) * d.lw + lx;
+    }
+    if ((d.status & STATUS_MASK) != LEAF_OK) {
+        if (inside) out[dst] = __builtin_nanf("");
+        if (tid == 0) atomicOr(errFlag, 1u);
+        return;
+    }
+    const SciHeader h = sci_header(d.sciHeader);
+    const Bytes b{a.payload + (size_t)d.word * 4u};
+    if (tid < 64u) ctrl[tid] = sci_control(a, b, h, tid, t);
+    __syncthreads();
+    if (!inside) return;
+    out[dst] = sci_coarse_sample(ctrl, lx, ly, lz) + sci_residual(a, b, h, lx, ly, lz, (uint32_t)t);
+}
+
+// ------------------------------------------------------ render voxels --
+struct ComposeDev {
+    int signRule;
+    float background, shellEmptyScale, shellEmptyMax;
+};
+
+__device__ inline float sample_grid(const float* g, int res, int lw, int lh, int ld, int lx, int ly, int lz)
+{
+    if (res < 1) return 0.0f;
+    int x0, x1, y0, y1, z0, z1;
+    float tx, ty, tz;
+    locate((float)lx, lw, res, x0, x1, tx);
+    locate((float)ly, lh, res, y0, y1, ty);
+    locate((float)lz, ld, res, z0, z1, tz);
+    float s[8];
+    s[0] = g[(z0 * res + y0) * res + x0];
+    s[1] = g[(z0 * res + y0) * res + x1];
+    s[2] = g[(z0 * res + y1) * res + x0];
+    s[3] = g[(z0 * res + y1) * res + x1];
+    s[4] = g[(z1 * res + y0) * res + x0];
+    s[5] = g[(z1 * res + y0) * res + x1];
+    s[6] = g[(z1 * res + y1) * res + x0];
+    s[7] = g[(z1 * res + y1) * res + x1];
+    return trilinear(s, tx, ty, tz);
+}
+
+__device__ inline float apply_sign(const ComposeDev& cm, float v, bool occupied)
+{
+    if (cm.signRule == VBT_SIGN_NONE) {
+        if (!isnan(cm.shellEmptyMax) && cm.shellEmptyScale < 1.0f && !occupied && v > 0.0f && v <= cm.shellEmptyMax)
+            v *= cm.shellEmptyScale;
+    } else if (!occupied && v < 0.0f) {
+        v = cm.signRule == VBT_SIGN_CLAMP_ZERO ? 0.0f : cm.background;
+    }
+    return v;
+}
+
+// shell residual of a voxel (decodeRenderTemporalLeafShellVoxelValue semantics)
+__device__ inline float shell_value(const Bytes& b, const DevLeaf& d, uint32_t vox, int F, int t, bool& occupied)
+{
+    const uint32_t di = shell_desc_index(b, d.plan.occOffset, vox);
+    occupied = di != NO_LEAF;
+    if (!occupied || d.plan.shell.keys == 0u) return 0.0f;
+    return decode_control(b, d.plan.shell, di, F, t);
+}
+
+// whole frame, render route: one workgroup per leaf, 512 lanes = voxels
+__global__ void __launch_bounds__(512) k_dense_render(AssetDev a, int t, ComposeDev cm, float* out, uint32_t* errFlag)
+{
+    __shared__ float coarse[343];
+    __shared__ float fine[343];
+    const uint32_t leaf = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    uint32_t bx, by, bz, lw, lh, ld;
+    leaf_coords(leaf, a.W, a.H, a.D, bx, by, bz, lw, lh, ld);
+    const uint32_t lx = tid & 7u, ly = (tid >> 3) & 7u, lz = tid >> 6;
+    const bool inside = lx < lw && ly < lh && lz < ld;
+    const uint64_t dst = ((uint64_t)(bz * 8u + lz) * a.H + (by * 8u + ly)) * a.W + bx * 8u + lx;
+    const uint32_t c = a.compact[leaf];
+    if (c == NO_LEAF) {
+        if (inside) out[dst] = 0.0f;
+        return;
+    }
+    const DevLeaf& d = a.leaves[c];
+    if ((d.status & STATUS_MASK) != LEAF_OK) {
+        if (inside) out[dst] = __builtin_nanf("");
+        if (tid == 0) atomicOr(errFlag, 1u);
+        return;
+    }
+    const Bytes b{a.payload + (size_t)d.word * 4u};
+    const int F = (int)a.F;
+    const uint32_t cc = control_count(d.plan.coarseRes);
+    const uint32_t fc = d.plan.hasFine ? control_count(d.plan.fineRes) : 0u;
+    for (uint32_t i = tid; i < cc; i += blockDim.x) coarse[i] = decode_control(b, d.plan.coarse, i, F, t);
+    for (uint32_t i = tid; i < fc; i += blockDim.x) fine[i] = decode_control(b, d.plan.fine, i, F, t);
+    __syncthreads();
+    if (!inside) return;
+    float v = sample_grid(coarse, (int)d.plan.coarseRes, (int)lw, (int)lh, (int)ld, (int)lx, (int)ly, (int)lz);
+    const uint32_t vox = (lz * 8u + ly) * 8u + lx;
+    bool occupied = false;
+    if (fc > 0u) {
+        v += sample_grid(fine, (int)d.plan.fineRes, (int)lw, (int)lh, (int)ld, (int)lx, (int)ly, (int)lz);
+        if (d.plan.mode == MODE_SHELL) occupied = shell_desc_index(b, d.plan.occOffset, vox) != NO_LEAF;
+    } else if (d.plan.mode == MODE_SHELL) {
+        v += shell_value(b, d, vox, F, t, occupied);
+    }
+    if (d.plan.mode == MODE_SHELL) v = apply_sign(cm, v, occupied);
+    out[dst] = v;
+}
+
+// ----------------------------------------------------------- point queries --
+// render point value (8 corners per grid decoded by the 8 lanes of a group)
+template <int LANES>
+__device__ inline float render_point(const AssetDev& a, const DevLeaf& d, const ComposeDev& cm, uint32_t lx, uint32_t ly,
+                                     uint32_t lz, int t, uint32_t sub)
+{
+    const Bytes b{a.payload + (size_t)d.word * 4u};
+    const int F = (int)a.F;
+    float v = 0.0f;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int res = pass == 0 ? (int)d.plan.coarseRes : (int)d.plan.fineRes;
+        if (pass == 1 && !d.plan.hasFine) break;
+        if (res < 1) continue;
+        const StreamRef& s = pass == 0 ? d.plan.coarse : d.plan.fine;
+        int x0, x1, y0, y1, z0, z1;
+        float tx, ty, tz;
+        locate((float)lx, (int)d.lw, res, x0, x1, tx);
+        locate((float)ly, (int)d.lh, res, y0, y1, ty);
+        locate((float)lz, (int)d.ld, res, z0, z1, tz);
+        float sv[8];
+        if (LANES == 8) {
+            const int cx = (sub & 1u) ? x1 : x0, cy = (sub & 2u) ? y1 : y0, cz = (sub & 4u) ? z1 : z0;
+            const float mine = decode_control(b, s, (uint32_t)((cz * res + cy) * res + cx), F, t);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sv[i] = __shfl(mine, i, 8);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int cx = (i & 1) ? x1 : x0, cy = (i & 2) ? y1 : y0, cz = (i & 4) ? z1 : z0;
+                sv[i] = decode_control(b, s, (uint32_t)((cz * res + cy) * res + cx), F, t);
+            }
+        }
+        const float g = trilinear(sv, tx, ty, tz);
+        v = pass == 0 ? g : v + g;
+    }
+    if (d.plan.mode == MODE_SHELL) {
+        const uint32_t vox = (lz * 8u + ly) * 8u + lx;
+        bool occ = false;
+        if (!d.plan.hasFine) {
+            v += shell_value(b, d, vox, F, t, occ);
+        } else {
+            occ = shell_desc_index(b, d.plan.occOffset, vox) != NO_LEAF;
+        }
+        v = apply_sign(cm, v, occ);
+    }
+    return v;
+}
+
+template <int LANES>
+__device__ inline float sci_point(const AssetDev& a, const DevLeaf& d, uint32_t lx, uint32_t ly, uint32_t lz, int t,
+                                  uint32_t sub)
+{
+    const Bytes b{a.payload + (size_t)d.word * 4u};
+    const SciHeader h = sci_header(d.sciHeader);
+    int x0, y0, z0;
+    float tx, ty, tz;
+    sci_coarse_coord(lx, x0, tx);
+    sci_coarse_coord(ly, y0, ty);
+    sci_coarse_coord(lz, z0, tz);
+    float sv[8];
+    if (LANES == 8) {
+        const int cx = x0 + (int)(sub & 1u), cy = y0 + (int)((sub >> 1) & 1u), cz = z0 + (int)((sub >> 2) & 1u);
+        const float mine = sci_control(a, b, h, (uint32_t)((cz * 4 + cy) * 4 + cx), t);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) sv[i] = __shfl(mine, i, 8);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
+            sv[i] = sci_control(a, b, h, (uint32_t)((cz * 4 + cy) * 4 + cx), t);
+        }
+    }
+    return trilinear(sv, tx, ty, tz) + sci_residual(a, b, h, lx, ly, lz, (uint32_t)t);
+}
+
+template <int LANES>
+__global__ void __launch_bounds__(256) k_query(AssetDev a, const uint4* q, uint32_t n, ComposeDev cm, float* out,
+                                               int32_t* status, uint32_t* errFlag)
+{
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t qi = gid / LANES;
+    const uint32_t sub = (uint32_t)(gid % LANES);
+    // groups past the end still take part in the 8-lane shuffles, with a clamped query
+    const bool valid = qi < n;
+    const uint4 r = q[valid ? qi : (n - 1u)];
+    int32_t st = 0;
+    float v = 0.0f;
+    if (r.x >= a.W || r.y >= a.H || r.z >= a.D) {
+        st = VBT_QUERY_ERR_OUTSIDE;
+    } else if (a.profile == 2u && r.w >= a.F) {
+        st = VBT_QUERY_ERR_FRAME;
+    } else {
+        const uint32_t lcx = (a.W + 7u) / 8u, lcy = (a.H + 7u) / 8u;
+        const uint32_t leaf = ((r.z / 8u) * lcy + r.y / 8u) * lcx + r.x / 8u;
+        const uint32_t c = a.compact[leaf];
+        const int t = r.w > 0x7fffffffu ? 0x7fffffff : (int)r.w;
+        if (c != NO_LEAF) {
+            const DevLeaf& d = a.leaves[c];
+            if ((d.status & STATUS_MASK) != LEAF_OK) {
+                st = (int32_t)(d.status & STATUS_MASK);
+            } else if (a.profile == 2u) {
+                v = sci_point<LANES>(a, d, r.x & 7u, r.y & 7u, r.z & 7u, t, sub);
+            } else {
+                v = render_point<LANES>(a, d, cm, r.x & 7u, r.y & 7u, r.z & 7u, t, sub);
+            }
+        }
+    }
+    if (valid && sub == 0u) {
+        out[qi] = st ? __builtin_nanf("") : v;
+        if (status) status[qi] = st;
+        if (st) atomicOr(errFlag, 1u);
+    }
+}
+
+// ------------------------------------------- per-value decode (batched) --
+__device__ inline int32_t stream_checks(uint32_t count, uint32_t bins, uint32_t keys, uint32_t codec, uint32_t bytes,
+                                        int align, uint32_t ci, uint32_t& keyBase)
+{
+    if (ci >= count) return VBT_ERR_CONTROL_INDEX;
+    const uint32_t db = count * 4u;
+    uint32_t bb = count * bins, kb = keys * keyframe_bytes(codec);
+    if (align) {
+        bb = align_up4(bb);
+        kb = align_up4(kb);
+    }
+    if (db + bb + kb > bytes) return LEAF_ERR_STREAM_OOB;
+    keyBase = db + bb;
+    return 0;
+}
+
+__device__ inline float value_of_stream(const uint8_t* base, uint32_t codec, uint32_t count, uint32_t bins,
+                                        uint32_t keys, uint32_t bytes, int align, int F, uint32_t ci, int t,
+                                        int32_t& st)
+{
+    uint32_t keyBase = 0;
+    st = stream_checks(count, bins, keys, codec, bytes, align, ci, keyBase);
+    if (st) return __builtin_nanf("");
+    const Bytes b{base};
+    const uint32_t ks = b.u16(ci * 4u), kc = b.u8(ci * 4u + 2u);
+    if (kc == 0u) return 0.0f;
+    if (ks + kc > keys) {
+        st = LEAF_ERR_KEY_RANGE;
+        return __builtin_nanf("");
+    }
+    return decode_sequence(b, keyBase, codec, ks, kc, count * 4u + ci * bins, bins, F, t);
+}
+
+__global__ void k_control_values(const uint8_t* stream, uint32_t codec, uint32_t count, uint32_t bins, uint32_t keys,
+                                 uint32_t bytes, int align, int F, const uint16_t* ci, const int32_t* t, uint32_t n,
+                                 float* out, int32_t* status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t st;
+    out[i] = value_of_stream(stream, codec, count, bins, keys, bytes, align, F, ci[i], t[i], st);
+    if (status) status[i] = st;
+}
+
+// decodeRenderTemporalLeafControlValue after parseRenderTemporalPackedLeaf:
+// the stream is chosen from the parse layout (EMPTY leaves have a zero layout)
+__global__ void k_leaf_control_values(const uint8_t* leaf, uint32_t leafBytes, int useFine, int F, const uint16_t* ci,
+                                      const int32_t* t, uint32_t n, float* out, int32_t* status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Bytes b{leaf};
+    LeafPlan pl;
+    int32_t st = (int32_t)plan_leaf(b, leafBytes, pl);
+    // parse-level failures only (descriptor ranges are checked per control below)
+    if (st == LEAF_ERR_STREAM_OOB) st = 0;
+    float v = __builtin_nanf("");
+    if (st == 0) {
+        const Header h = unpack_header(b.u32(0));
+        const uint32_t cc = control_count(h.coarseRes), fc = control_count(h.fineRes);
+        uint32_t off = 0, bytes = 0, keys = 0, count = 0, codec = 0;
+        if (h.mode != MODE_EMPTY) {
+            const uint32_t ckc = b.u16(4), fkc = b.u16(6);
+            const uint32_t coarseBytes = stream_total_bytes(cc, 8u, ckc, h.coarseCodec);
+            if (useFine) {
+                off = 8u + coarseBytes;
+                count = fc;
+                keys = fkc;
+                codec = h.fineCodec;
+                bytes = h.mode == MODE_FINE6   ? stream_total_bytes(fc, 8u, fkc, h.fineCodec)
+                        : h.mode == MODE_SHELL ? 80u + align_up4(fkc * keyframe_bytes(h.fineCodec))
+                                               : 0u;
+            } else {
+                off = 8u;
+                count = cc;
+                keys = ckc;
+                codec = h.coarseCodec;
+                bytes = coarseBytes;
+            }
+        } else {
+            count = useFine ? fc : cc;
+            codec = useFine ? h.fineCodec : h.coarseCodec;
+        }
+        v = value_of_stream(leaf + off, codec, count, 8u, keys, bytes, 1, F, ci[i], t[i], st);
+    }
+    out[i] = v;
+    if (status) status[i] = st;
+}
+
+__global__ void k_shell_values(const uint8_t* leaf, uint32_t leafBytes, int F, const uint16_t* vox, const int32_t* t,
+                               uint32_t n, float* out, int32_t* status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Bytes b{leaf};
+    LeafPlan pl;
+    int32_t st = (int32_t)plan_leaf(b, leafBytes, pl);
+    if (st == LEAF_ERR_STREAM_OOB) st = 0;
+    float v = 0.0f;
+    if (st == 0 && pl.mode == MODE_SHELL && vox[i] < 512u) {
+        const uint32_t di = shell_desc_index(b, pl.occOffset, vox[i]);
+        if (di != NO_LEAF && pl.shell.count != 0u && pl.shell.keys != 0u)
+            v = value_of_stream(leaf + pl.shell.base, pl.shell.codec, pl.shell.count, 4u, pl.shell.keys, pl.shell.bytes,
+                                1, F, di, t[i], st);
+    } else if (st != 0) {
+        v = __builtin_nanf("");
+    }
+    out[i] = v;
+    if (status) status[i] = st;
+}
+
+__global__ void k_fill(float* p, uint64_t n, float v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+inline hipStream_t S(void* s) { return (hipStream_t)s; }
+
+inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+} // namespace
+
+// ============================================================== assets ==
+struct vbt_asset_s {
+    vbt_file_header h{};
+    int device = 0;
+    uint32_t* d_offsets = nullptr;
+    uint32_t* d_payload = nullptr;
+    float* d_scales = nullptr;
+    uint32_t* d_compact = nullptr;
+    uint32_t* d_status = nullptr;
+    DevLeaf* d_leaves = nullptr;
+    uint64_t* d_ctrl = nullptr;
+    float* d_basis = nullptr;
+    uint32_t* d_err = nullptr;   // error flag word for launches
+    Pair* d_blocks = nullptr;
+    uint32_t A = 0, bad = 0;
+    uint64_t controlsPerFrame = 0, payloadBytes = 0, deviceBytes = 0, fast = 0;
+    uint64_t modeCounts[4] = {0, 0, 0, 0};
+
+    AssetDev dev() const
+    {
+        AssetDev a;
+        a.offsets = d_offsets;
+        a.payload = (const uint8_t*)d_payload;
+        a.scales = d_scales;
+        a.compact = d_compact;
+        a.leaves = d_leaves;
+        a.ctrl = d_ctrl;
+        a.basis = d_basis;
+        a.W = h.width;
+        a.H = h.height;
+        a.D = h.depth;
+        a.F = h.frames;
+        a.L = h.leafCount;
+        a.A = A;
+        a.profile = h.profileType;
+        a.scaleCount = h.coarseAcScaleCount;
+        a.payloadBytes = payloadBytes;
+        return a;
+    }
+};
+
+namespace {
+
+template <class T> int dalloc(vbt_asset a, T** p, size_t count)
+{
+    const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 4;
+    HIPCHK(hipMalloc((void**)p, bytes));
+    a->deviceBytes += bytes;
+    return VBT_OK;
+}
+
+void free_asset(vbt_asset a)
+{
+    if (!a) return;
+    void* ptrs[] = {a->d_offsets, a->d_payload, a->d_scales, a->d_compact, a->d_status,
+                    a->d_leaves,  a->d_ctrl,    a->d_basis,  a->d_err,     a->d_blocks};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete a;
+}
+
+int check_err_flag(vbt_asset a, hipStream_t s, const char* what)
+{
+    uint32_t flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, a->d_err, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (flag) return vbt_fail(VBT_ERR_FORMAT, std::string(what) + ": request touches a leaf that failed validation");
+    return VBT_OK;
+}
+
+ComposeDev to_dev(const vbt_compose* c)
+{
+    ComposeDev d{VBT_SIGN_NONE, 0.0f, 1.0f, NAN};
+    if (c) {
+        d.signRule = c->signRule;
+        d.background = c->background;
+        d.shellEmptyScale = c->shellEmptyScale;
+        d.shellEmptyMax = c->shellEmptyMax;
+    }
+    return d;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* vbt_last_error(void) { return g_err.c_str(); }
+
+const char* vbt_version(void) { return "vbt_hip 1.0 (gfx950)"; }
+
+int vbt_device_count(int* n)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return VBT_OK;
+}
+
+int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_asset* out)
+{
+    *out = nullptr;
+    vbt_file_header h;
+    int rc = vbt_validate_file(bytes, size, &h);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return vbt_fail(VBT_ERR_NO_DEVICE, "no HIP device");
+    hipStream_t s = S(stream);
+    vbt_asset a = new vbt_asset_s();
+    a->h = h;
+    HIPCHK(hipGetDevice(&a->device));
+    const uint8_t* base = (const uint8_t*)bytes;
+    const uint64_t scaleOff = 52, offOff = 52 + 4ull * h.coarseAcScaleCount;
+    const uint64_t payOff = offOff + 4ull * (h.leafCount + 1ull);
+    a->payloadBytes = size - payOff;
+    const uint32_t L = h.leafCount;
+    const uint32_t nb = blocks_for(L, 256);
+    if ((rc = dalloc(a, &a->d_offsets, (size_t)L + 1)) || (rc = dalloc(a, &a->d_payload, (a->payloadBytes + 3) / 4 + 1)) ||
+        (rc = dalloc(a, &a->d_scales, h.coarseAcScaleCount + 1)) || (rc = dalloc(a, &a->d_compact, L)) ||
+        (rc = dalloc(a, &a->d_status, L)) || (rc = dalloc(a, &a->d_err, 4)) || (rc = dalloc(a, &a->d_blocks, nb + 1))) {
+        free_asset(a);
+        return rc;
+    }
+    auto fail = [&](int code) {
+        free_asset(a);
+        return code;
+    };
+#define TRY(x)                                                                                          \
+    do {                                                                                                \
+        hipError_t e_ = (x);                                                                            \
+        if (e_ != hipSuccess) {                                                                         \
+            vbt_fail(e_ == hipErrorOutOfMemory ? VBT_ERR_NOMEM : VBT_ERR_DEVICE,                        \
+                     std::string(#x " failed: ") + hipGetErrorString(e_));                              \
+            return fail(e_ == hipErrorOutOfMemory ? VBT_ERR_NOMEM : VBT_ERR_DEVICE);                    \
+        }                                                                                               \
+    } while (0)
+    TRY(hipMemcpyAsync(a->d_offsets, base + offOff, 4ull * (L + 1ull), hipMemcpyHostToDevice, s));
+    if (a->payloadBytes) TRY(hipMemcpyAsync(a->d_payload, base + payOff, a->payloadBytes, hipMemcpyHostToDevice, s));
+    if (h.coarseAcScaleCount)
+        TRY(hipMemcpyAsync(a->d_scales, base + scaleOff, 4ull * h.coarseAcScaleCount, hipMemcpyHostToDevice, s));
+    if (h.profileType == 2) {
+        // dctBasis table (render/src/scientific_decode.cpp:86-94): computed in double, stored as float
+        std::vector<float> basis((size_t)h.frames * 64);
+        for (uint32_t t = 0; t < h.frames; ++t)
+            for (int k = 0; k < 64; ++k) {
+                const double invN = 1.0 / (double)h.frames;
+                const double alpha = (k == 0) ? std::sqrt(invN) : std::sqrt(2.0 * invN);
+                basis[(size_t)t * 64 + k] = (float)(alpha * std::cos((3.14159265358979323846 / (double)h.frames) *
+                                                                     ((double)t + 0.5) * (double)k));
+            }
+        if ((rc = dalloc(a, &a->d_basis, basis.size()))) return fail(rc);
+        TRY(hipMemcpyAsync(a->d_basis, basis.data(), basis.size() * 4, hipMemcpyHostToDevice, s));
+        TRY(hipStreamSynchronize(s));
+    }
+    AssetDev ad = a->dev();
+    Pair* d_total = a->d_blocks + nb;
+    k_scan_a<<<nb, 256, 0, s>>>(ad, a->d_blocks);
+    k_scan_b<<<1, 1024, 0, s>>>(a->d_blocks, nb, d_total);
+    TRY(hipGetLastError());
+    Pair total;
+    TRY(hipMemcpyAsync(&total, d_total, sizeof(Pair), hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    a->A = total.active;
+    a->controlsPerFrame = total.n;
+    if ((rc = dalloc(a, &a->d_leaves, a->A + 1)) || (rc = dalloc(a, &a->d_ctrl, (size_t)a->A + 1))) return fail(rc);
+    ad = a->dev();
+    k_scan_c<<<nb, 256, 0, s>>>(ad, a->d_blocks, a->d_compact, a->d_status, a->d_leaves, a->d_ctrl);
+    TRY(hipGetLastError());
+    TRY(hipMemcpyAsync(a->d_ctrl + a->A, &a->controlsPerFrame, 8, hipMemcpyHostToDevice, s));
+    if (a->A) k_validate<<<a->A, 256, 0, s>>>(ad, a->d_leaves, a->d_status);
+    TRY(hipGetLastError());
+    // host-side statistics of the page table
+    std::vector<uint32_t> st(L);
+    TRY(hipMemcpyAsync(st.data(), a->d_status, 4ull * L, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    const uint32_t* offs = (const uint32_t*)(base + offOff);
+    for (uint32_t l = 0; l < L; ++l) {
+        const uint32_t w = offs[l];
+        uint32_t hw;
+        std::memcpy(&hw, base + payOff + 4ull * w, 4);
+        a->modeCounts[hw & 3u]++;
+        if ((st[l] & STATUS_MASK) != LEAF_OK) a->bad++;
+        if (st[l] & FLAG_SWEEP) a->fast++;
+    }
+#undef TRY
+    *out = a;
+    return VBT_OK;
+}
+
+int vbt_asset_open_file(const char* path, void* stream, vbt_asset* out)
+{
+    std::ifstream in(path, std::ios::binary | std::ios::ate);
+    if (!in) return vbt_fail(VBT_ERR_INVALID, std::string("Failed to open file: ") + path);
+    const std::streamsize n = in.tellg();
+    std::vector<uint32_t> buf(((size_t)n + 3) / 4);
+    in.seekg(0);
+    in.read((char*)buf.data(), n);
+    if (!in) return vbt_fail(VBT_ERR_INVALID, std::string("Failed to read file: ") + path);
+    return vbt_asset_open_memory(buf.data(), (uint64_t)n, stream, out);
+}
+
+void vbt_asset_close(vbt_asset a) { free_asset(a); }
+
+int vbt_asset_info_get(vbt_asset a, vbt_asset_info* o)
+{
+    if (!a || !o) return vbt_fail(VBT_ERR_INVALID, "null asset");
+    std::memset(o, 0, sizeof(*o));
+    o->header = a->h;
+    o->activeLeaves = a->A;
+    o->badLeaves = a->bad;
+    o->controlsPerFrame = a->controlsPerFrame;
+    o->payloadBytes = a->payloadBytes;
+    o->deviceBytes = a->deviceBytes;
+    for (int i = 0; i < 4; ++i) o->mode_counts[i] = a->modeCounts[i];
+    o->fast_leaves = a->fast;
+    return VBT_OK;
+}
+
+int vbt_asset_page_table(vbt_asset a, uint32_t* d_compact, void* stream)
+{
+    HIPCHK(hipMemcpyAsync(d_compact, a->d_compact, 4ull * a->h.leafCount, hipMemcpyDeviceToDevice, S(stream)));
+    return VBT_OK;
+}
+
+int vbt_asset_cache_offsets(vbt_asset a, uint64_t* d_offsets, void* stream)
+{
+    HIPCHK(hipMemcpyAsync(d_offsets, a->d_ctrl, 8ull * (a->A + 1ull), hipMemcpyDeviceToDevice, S(stream)));
+    return VBT_OK;
+}
+
+int vbt_asset_leaf_status(vbt_asset a, uint32_t* d_status, void* stream)
+{
+    HIPCHK(hipMemcpyAsync(d_status, a->d_status, 4ull * a->h.leafCount, hipMemcpyDeviceToDevice, S(stream)));
+    return VBT_OK;
+}
+
+int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out, void* stream)
+{
+    if (!a || !d_out || nt <= 0) return vbt_fail(VBT_ERR_INVALID, "invalid frame cache request");
+    if (t0 < 0 || (uint64_t)t0 + (uint64_t)nt > a->h.frames)
+        return vbt_fail(VBT_ERR_INVALID, "frame range outside the asset");
+    if (layout != VBT_CACHE_PACKED && layout != VBT_CACHE_STRIDE576)
+        return vbt_fail(VBT_ERR_INVALID, "unknown cache layout");
+    if (a->h.profileType == 2 && layout != VBT_CACHE_PACKED)
+        return vbt_fail(VBT_ERR_INVALID, "scientific assets support the packed cache only");
+    hipStream_t s = S(stream);
+    k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
+    const AssetDev ad = a->dev();
+    if (a->A == 0) return VBT_OK;
+    if (a->h.profileType == 2) {
+        k_sci_leaf<false><<<dim3(a->A, (uint32_t)nt), 512, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+    } else if (layout == VBT_CACHE_PACKED) {
+        k_cache_render<VBT_CACHE_PACKED>
+            <<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+    } else {
+        const uint64_t n = (uint64_t)a->A * 576u * (uint64_t)nt;
+        k_fill<<<(uint32_t)std::min<uint64_t>(blocks_for(n, 256), 65535), 256, 0, s>>>(d_out, n, 0.0f);
+        k_cache_render<VBT_CACHE_STRIDE576>
+            <<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+    }
+    HIPCHK(hipGetLastError());
+    if (a->bad) return check_err_flag(a, s, "frame cache");
+    return VBT_OK;
+}
+
+int vbt_decode_frame_dense(vbt_asset a, int t, const vbt_compose* c, float* d_out, void* stream)
+{
+    if (!a || !d_out) return vbt_fail(VBT_ERR_INVALID, "invalid dense frame request");
+    if (a->h.profileType == 2 && (t < 0 || (uint32_t)t >= a->h.frames))
+        return vbt_fail(VBT_ERR_INVALID, "scientific frame index out of range");
+    hipStream_t s = S(stream);
+    k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
+    const AssetDev ad = a->dev();
+    if (a->h.profileType == 2) k_sci_leaf<true><<<dim3(a->A, 1), 512, 0, s>>>(ad, t, 1, d_out, 0, a->d_err);
+    else k_dense_render<<<a->h.leafCount, 512, 0, s>>>(ad, t, to_dev(c), d_out, a->d_err);
+    HIPCHK(hipGetLastError());
+    if (a->bad) return check_err_flag(a, s, "dense frame");
+    return VBT_OK;
+}
+
+static int launch_query(vbt_asset a, const uint32_t* d_q, uint32_t n, const vbt_compose* c, float* d_out,
+                        int32_t* d_status, void* stream, bool plain)
+{
+    if (!a || !d_q || !d_out) return vbt_fail(VBT_ERR_INVALID, "invalid query request");
+    if (n == 0) return VBT_OK;
+    hipStream_t s = S(stream);
+    k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
+    const AssetDev ad = a->dev();
+    if (plain) k_query<1><<<blocks_for(n, 256), 256, 0, s>>>(ad, (const uint4*)d_q, n, to_dev(c), d_out, d_status, a->d_err);
+    else k_query<8><<<blocks_for(8ull * n, 256), 256, 0, s>>>(ad, (const uint4*)d_q, n, to_dev(c), d_out, d_status, a->d_err);
+    HIPCHK(hipGetLastError());
+    return VBT_OK;
+}
+
+int vbt_query_points(vbt_asset a, const uint32_t* d_q, uint32_t n, const vbt_compose* c, float* d_out,
+                     int32_t* d_status, void* stream)
+{
+    return launch_query(a, d_q, n, c, d_out, d_status, stream, false);
+}
+
+int vbt_query_points_plain(vbt_asset a, const uint32_t* d_q, uint32_t n, const vbt_compose* c, float* d_out,
+                           int32_t* d_status, void* stream)
+{
+    return launch_query(a, d_q, n, c, d_out, d_status, stream, true);
+}
+
+int vbt_decode_control_values(const uint8_t* d_stream, uint32_t codec, uint16_t controlCount, uint8_t timeBinCount,
+                              uint8_t maxBinLocalKeys, uint32_t keyframeCount, uint32_t streamBytes, int align,
+                              int totalFrames, const uint16_t* d_ci, const int32_t* d_t, uint32_t n, float* d_out,
+                              int32_t* d_status, void* stream)
+{
+    (void)maxBinLocalKeys;
+    if (n == 0) return VBT_OK;
+    k_control_values<<<blocks_for(n, 256), 256, 0, S(stream)>>>(d_stream, codec, controlCount, timeBinCount,
+                                                                keyframeCount, streamBytes, align, totalFrames, d_ci,
+                                                                d_t, n, d_out, d_status);
+    HIPCHK(hipGetLastError());
+    return VBT_OK;
+}
+
+int vbt_decode_leaf_control_values(const uint8_t* d_leaf, uint32_t leafBytes, int useFine, int align, int totalFrames,
+                                   const uint16_t* d_ci, const int32_t* d_t, uint32_t n, float* d_out,
+                                   int32_t* d_status, void* stream)
+{
+    if (!align) return vbt_fail(VBT_ERR_INVALID, "the device leaf decoders read 4-byte aligned leaves");
+    if (n == 0) return VBT_OK;
+    k_leaf_control_values<<<blocks_for(n, 256), 256, 0, S(stream)>>>(d_leaf, leafBytes, useFine, totalFrames, d_ci,
+                                                                     d_t, n, d_out, d_status);
+    HIPCHK(hipGetLastError());
+    return VBT_OK;
+}
+
+int vbt_decode_shell_voxel_values(const uint8_t* d_leaf, uint32_t leafBytes, int align, int totalFrames,
+                                  const uint16_t* d_vox, const int32_t* d_t, uint32_t n, float* d_out,
+                                  int32_t* d_status, void* stream)
+{
+    if (!align) return vbt_fail(VBT_ERR_INVALID, "the device leaf decoders read 4-byte aligned leaves");
+    if (n == 0) return VBT_OK;
+    k_shell_values<<<blocks_for(n, 256), 256, 0, S(stream)>>>(d_leaf, leafBytes, totalFrames, d_vox, d_t, n, d_out,
+                                                              d_status);
+    HIPCHK(hipGetLastError());
+    return VBT_OK;
+}
+
+// ----------------------------------------------------------- device utils --
+int vbt_device_alloc(void** p, size_t bytes)
+{
+    HIPCHK(hipMalloc(p, bytes ? bytes : 4));
+    return VBT_OK;
+}
+int vbt_device_free(void* p)
+{
+    if (p) HIPCHK(hipFree(p));
+    return VBT_OK;
+}
+int vbt_memcpy_h2d(void* d, const void* s, size_t n, void* stream)
+{
+    HIPCHK(hipMemcpyAsync(d, s, n, hipMemcpyHostToDevice, S(stream)));
+    return VBT_OK;
+}
+int vbt_memcpy_d2h(void* d, const void* s, size_t n, void* stream)
+{
+    HIPCHK(hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, S(stream)));
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    return VBT_OK;
+}
+int vbt_memset_d(void* d, int v, size_t n, void* stream)
+{
+    HIPCHK(hipMemsetAsync(d, v, n, S(stream)));
+    return VBT_OK;
+}
+int vbt_stream_create(void** s)
+{
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *s = st;
+    return VBT_OK;
+}
+int vbt_stream_destroy(void* s)
+{
+    if (s) HIPCHK(hipStreamDestroy(S(s)));
+    return VBT_OK;
+}
+int vbt_stream_synchronize(void* s)
+{
+    HIPCHK(hipStreamSynchronize(S(s)));
+    return VBT_OK;
+}
+int vbt_event_create(void** e)
+{
+    hipEvent_t ev;
+    HIPCHK(hipEventCreate(&ev));
+    *e = ev;
+    return VBT_OK;
+}
+int vbt_event_destroy(void* e)
+{
+    if (e) HIPCHK(hipEventDestroy((hipEvent_t)e));
+    return VBT_OK;
+}
+int vbt_event_record(void* e, void* s)
+{
+    HIPCHK(hipEventRecord((hipEvent_t)e, S(s)));
+    return VBT_OK;
+}
+int vbt_event_elapsed_ms(void* a, void* b, float* ms)
+{
+    HIPCHK(hipEventSynchronize((hipEvent_t)b));
+    HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return VBT_OK;
+}
+
+} // extern "C"

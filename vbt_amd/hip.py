@@ -1,0 +1,336 @@
+"""Python host side of libvbt_hip.so (ctypes over the C-ABI in include/vbt_hip.h).
+
+Device memory and streams come from the library itself (vbt_device_alloc,
+vbt_stream_create), so no other GPU runtime is needed in the process.  Every
+call raises VbtError with the library's message on failure; the library is
+required -- there is no fallback decode.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvbt_hip.so")
+
+VBT_OK = 0
+CACHE_PACKED, CACHE_STRIDE576 = 0, 1
+SIGN_NONE, SIGN_CLAMP_ZERO, SIGN_BACKGROUND = 0, 1, 2
+
+c_void_p = ctypes.c_void_p
+c_u32 = ctypes.c_uint32
+c_u64 = ctypes.c_uint64
+c_int = ctypes.c_int
+
+
+class VbtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[vbt {code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class FileHeader(ctypes.Structure):
+    _fields_ = [(n, c_u32) for n in ("version", "width", "height", "depth", "frames", "leafSize",
+                                       "coarseResolution", "maxCoarseKeep", "leafCount", "profileType",
+                                       "coarseAcScaleCount")]
+
+
+class AssetInfo(ctypes.Structure):
+    _fields_ = [("header", FileHeader), ("activeLeaves", c_u32), ("badLeaves", c_u32),
+                ("controlsPerFrame", c_u64), ("payloadBytes", c_u64), ("deviceBytes", c_u64),
+                ("mode_counts", c_u64 * 4), ("fast_leaves", c_u64)]
+
+
+class Compose(ctypes.Structure):
+    _fields_ = [("signRule", c_int), ("background", ctypes.c_float), ("shellEmptyScale", ctypes.c_float),
+                ("shellEmptyMax", ctypes.c_float)]
+
+
+def compose(sign_rule=SIGN_NONE, background=0.0, shell_empty_scale=1.0, shell_empty_max=math.nan) -> Compose:
+    return Compose(sign_rule, background, shell_empty_scale, shell_empty_max)
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.vbt_last_error.restype = ctypes.c_char_p
+        L.vbt_version.restype = ctypes.c_char_p
+        L.vbt_status_message.restype = ctypes.c_char_p
+        L.vbt_status_message.argtypes = [ctypes.c_int32]
+        L.vbt_device_alloc.argtypes = [ctypes.POINTER(c_void_p), ctypes.c_size_t]
+        L.vbt_device_free.argtypes = [c_void_p]
+        L.vbt_memcpy_h2d.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]
+        L.vbt_memcpy_d2h.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]
+        L.vbt_memset_d.argtypes = [c_void_p, c_int, ctypes.c_size_t, c_void_p]
+        L.vbt_stream_create.argtypes = [ctypes.POINTER(c_void_p)]
+        L.vbt_stream_destroy.argtypes = [c_void_p]
+        L.vbt_stream_synchronize.argtypes = [c_void_p]
+        L.vbt_event_create.argtypes = [ctypes.POINTER(c_void_p)]
+        L.vbt_event_destroy.argtypes = [c_void_p]
+        L.vbt_event_record.argtypes = [c_void_p, c_void_p]
+        L.vbt_event_elapsed_ms.argtypes = [c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.vbt_device_count.argtypes = [ctypes.POINTER(c_int)]
+        L.vbt_asset_open_memory.argtypes = [c_void_p, c_u64, c_void_p, ctypes.POINTER(c_void_p)]
+        L.vbt_asset_open_file.argtypes = [ctypes.c_char_p, c_void_p, ctypes.POINTER(c_void_p)]
+        L.vbt_asset_close.argtypes = [c_void_p]
+        L.vbt_asset_close.restype = None
+        L.vbt_asset_info_get.argtypes = [c_void_p, ctypes.POINTER(AssetInfo)]
+        L.vbt_asset_page_table.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.vbt_asset_cache_offsets.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.vbt_asset_leaf_status.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.vbt_decode_frame_cache.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]
+        L.vbt_decode_frame_dense.argtypes = [c_void_p, c_int, ctypes.POINTER(Compose), c_void_p, c_void_p]
+        L.vbt_query_points.argtypes = [c_void_p, c_void_p, c_u32, ctypes.POINTER(Compose), c_void_p, c_void_p,
+                                       c_void_p]
+        L.vbt_query_points_plain.argtypes = L.vbt_query_points.argtypes
+        L.vbt_decode_control_values.argtypes = [c_void_p, c_u32, ctypes.c_uint16, ctypes.c_uint8, ctypes.c_uint8,
+                                                c_u32, c_u32, c_int, c_int, c_void_p, c_void_p, c_u32, c_void_p,
+                                                c_void_p, c_void_p]
+        L.vbt_decode_leaf_control_values.argtypes = [c_void_p, c_u32, c_int, c_int, c_int, c_void_p, c_void_p, c_u32,
+                                                     c_void_p, c_void_p, c_void_p]
+        L.vbt_decode_shell_voxel_values.argtypes = [c_void_p, c_u32, c_int, c_int, c_void_p, c_void_p, c_u32,
+                                                    c_void_p, c_void_p, c_void_p]
+        L.vbt_validate_file.argtypes = [c_void_p, c_u64, ctypes.POINTER(FileHeader)]
+        L.vbt_half_bits_to_float.restype = ctypes.c_float
+        L.vbt_half_bits_to_float.argtypes = [ctypes.c_uint16]
+        L.vbt_float_to_half_bits.restype = ctypes.c_uint16
+        L.vbt_float_to_half_bits.argtypes = [ctypes.c_float]
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != VBT_OK:
+        raise VbtError(rc, lib().vbt_last_error().decode())
+
+
+def device_count() -> int:
+    n = c_int(0)
+    lib().vbt_device_count(ctypes.byref(n))
+    return n.value
+
+
+class Stream:
+    def __init__(self):
+        self.h = c_void_p()
+        check(lib().vbt_stream_create(ctypes.byref(self.h)))
+
+    def sync(self):
+        check(lib().vbt_stream_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().vbt_stream_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation with a numpy dtype/shape (host copies on demand)."""
+
+    def __init__(self, shape, dtype, stream: Stream | None = None):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = c_void_p()
+        self.stream = stream
+        check(lib().vbt_device_alloc(ctypes.byref(self.ptr), max(self.nbytes, 4)))
+
+    @classmethod
+    def from_host(cls, arr: np.ndarray, stream: Stream | None = None):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.shape, arr.dtype, stream)
+        check(lib().vbt_memcpy_h2d(d.ptr, arr.ctypes.data, arr.nbytes, stream.h if stream else None))
+        if stream:
+            stream.sync()
+        return d
+
+    def host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(lib().vbt_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, self.stream.h if self.stream else None))
+        return out
+
+    def fill_bytes(self, value: int):
+        check(lib().vbt_memset_d(self.ptr, value, self.nbytes, self.stream.h if self.stream else None))
+
+    def free(self):
+        if self.ptr:
+            lib().vbt_device_free(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.h = c_void_p()
+        check(lib().vbt_event_create(ctypes.byref(self.h)))
+
+    def record(self, stream: Stream):
+        check(lib().vbt_event_record(self.h, stream.h))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = ctypes.c_float()
+        check(lib().vbt_event_elapsed_ms(self.h, later.h, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().vbt_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Asset:
+    """A VBTPACK4 asset resident in HBM with its device page table."""
+
+    def __init__(self, data: bytes, stream: Stream | None = None):
+        self.stream = stream or Stream()
+        self._buf = np.frombuffer(bytearray(data), dtype=np.uint8)
+        self.h = c_void_p()
+        check(lib().vbt_asset_open_memory(self._buf.ctypes.data, len(data), self.stream.h, ctypes.byref(self.h)))
+        self._buf = None
+        self.info = AssetInfo()
+        check(lib().vbt_asset_info_get(self.h, ctypes.byref(self.info)))
+
+    @classmethod
+    def open_file(cls, path: str, stream: Stream | None = None):
+        self = cls.__new__(cls)
+        self.stream = stream or Stream()
+        self.h = c_void_p()
+        check(lib().vbt_asset_open_file(path.encode(), self.stream.h, ctypes.byref(self.h)))
+        self.info = AssetInfo()
+        check(lib().vbt_asset_info_get(self.h, ctypes.byref(self.info)))
+        return self
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vbt_asset_close(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def header(self):
+        return self.info.header
+
+    @property
+    def dims(self):
+        h = self.info.header
+        return h.width, h.height, h.depth, h.frames
+
+    def page_table(self) -> np.ndarray:
+        d = DeviceArray(self.header.leafCount, np.uint32, self.stream)
+        check(lib().vbt_asset_page_table(self.h, d.ptr, self.stream.h))
+        return d.host()
+
+    def cache_offsets(self) -> np.ndarray:
+        d = DeviceArray(self.info.activeLeaves + 1, np.uint64, self.stream)
+        check(lib().vbt_asset_cache_offsets(self.h, d.ptr, self.stream.h))
+        return d.host()
+
+    def leaf_status(self) -> np.ndarray:
+        d = DeviceArray(self.header.leafCount, np.uint32, self.stream)
+        check(lib().vbt_asset_leaf_status(self.h, d.ptr, self.stream.h))
+        return d.host()
+
+    def frame_cache_into(self, out: DeviceArray, t0: int, nt: int, layout=CACHE_PACKED):
+        check(lib().vbt_decode_frame_cache(self.h, t0, nt, layout, out.ptr, self.stream.h))
+
+    def frame_cache(self, t0: int, nt: int = 1, layout=CACHE_PACKED) -> np.ndarray:
+        n = self.info.controlsPerFrame if layout == CACHE_PACKED else self.info.activeLeaves * 576
+        out = DeviceArray((nt, n), np.float32, self.stream)
+        self.frame_cache_into(out, t0, nt, layout)
+        return out.host()
+
+    def dense_into(self, out: DeviceArray, t: int, comp: Compose | None = None):
+        comp = comp or compose()
+        check(lib().vbt_decode_frame_dense(self.h, t, ctypes.byref(comp), out.ptr, self.stream.h))
+
+    def dense(self, t: int, comp: Compose | None = None) -> np.ndarray:
+        W, H, D, _ = self.dims
+        out = DeviceArray((D, H, W), np.float32, self.stream)
+        self.dense_into(out, t, comp)
+        return out.host()
+
+    def query_into(self, q: DeviceArray, n: int, out: DeviceArray, comp: Compose | None = None,
+                   status: DeviceArray | None = None, plain=False):
+        comp = comp or compose()
+        fn = lib().vbt_query_points_plain if plain else lib().vbt_query_points
+        check(fn(self.h, q.ptr, n, ctypes.byref(comp), out.ptr, status.ptr if status else None, self.stream.h))
+
+    def query(self, q: np.ndarray, comp: Compose | None = None, plain=False, with_status=False):
+        q = np.ascontiguousarray(q, dtype=np.uint32).reshape(-1, 4)
+        n = len(q)
+        dq = DeviceArray.from_host(q, self.stream)
+        out = DeviceArray(n, np.float32, self.stream)
+        st = DeviceArray(n, np.int32, self.stream)
+        self.query_into(dq, n, out, comp, st, plain)
+        vals = out.host()
+        return (vals, st.host()) if with_status else vals
+
+
+def status_message(code: int) -> str:
+    return lib().vbt_status_message(int(code)).decode()
+
+
+def decode_control_values(stream_bytes: bytes, codec, control_count, bins, keyframe_count, total_frames, ci, t,
+                          align=1, stream_limit=None):
+    """Batched decodeRenderTemporalControlValue on the GPU (host arrays in/out)."""
+    s = Stream()
+    buf = np.frombuffer(bytes(stream_bytes) + b"\0\0\0\0", dtype=np.uint8)
+    d_stream = DeviceArray.from_host(buf, s)
+    ci = np.ascontiguousarray(ci, np.uint16)
+    t = np.ascontiguousarray(t, np.int32)
+    d_ci = DeviceArray.from_host(ci, s)
+    d_t = DeviceArray.from_host(t, s)
+    out = DeviceArray(len(ci), np.float32, s)
+    st = DeviceArray(len(ci), np.int32, s)
+    limit = len(stream_bytes) if stream_limit is None else stream_limit
+    check(lib().vbt_decode_control_values(d_stream.ptr, codec, control_count, bins, 0, keyframe_count, limit, align,
+                                          total_frames, d_ci.ptr, d_t.ptr, len(ci), out.ptr, st.ptr, s.h))
+    return out.host(), st.host()
+
+
+def decode_leaf_values(leaf_bytes: bytes, total_frames, idx, t, use_fine=False, shell=False):
+    """Batched decodeRenderTemporalLeafControlValue / ...ShellVoxelValue on the GPU."""
+    s = Stream()
+    buf = np.frombuffer(bytes(leaf_bytes) + b"\0\0\0\0", dtype=np.uint8)
+    d_leaf = DeviceArray.from_host(buf, s)
+    idx = np.ascontiguousarray(idx, np.uint16)
+    t = np.ascontiguousarray(t, np.int32)
+    d_i = DeviceArray.from_host(idx, s)
+    d_t = DeviceArray.from_host(t, s)
+    out = DeviceArray(len(idx), np.float32, s)
+    st = DeviceArray(len(idx), np.int32, s)
+    if shell:
+        check(lib().vbt_decode_shell_voxel_values(d_leaf.ptr, len(leaf_bytes), 1, total_frames, d_i.ptr, d_t.ptr,
+                                                  len(idx), out.ptr, st.ptr, s.h))
+    else:
+        check(lib().vbt_decode_leaf_control_values(d_leaf.ptr, len(leaf_bytes), int(use_fine), 1, total_frames,
+                                                   d_i.ptr, d_t.ptr, len(idx), out.ptr, st.ptr, s.h))
+    return out.host(), st.host()
