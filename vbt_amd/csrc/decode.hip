@@ -264,32 +264,40 @@ __global__ void __launch_bounds__(256) k_scan_c(AssetDev a, const Pair* blockBas
     leaves[c] = d;
 }
 
-// maximum payload byte (exclusive) a scientific leaf decode can read
+// one past the last payload byte a scientific leaf decode reads (the
+// reference reads through std::vector::at over the whole pool, so a read past
+// the leaf but inside the pool is legal; past the pool it throws)
 __device__ inline uint64_t sci_read_end(const AssetDev& a, const DevLeaf& d)
 {
     const SciHeader h = sci_header(d.sciHeader);
+    const Bytes b{a.payload};
     const uint64_t base = (uint64_t)d.word * 4u;
     uint64_t end = base + 4u + 64u * (h.keep + 1u);
-    if (h.mode == 1u) {
+    if (h.mode == 1u && h.count > 0u) {
         const uint32_t cap = h.tier == 0 ? 64u : h.tier == 1 ? 256u : h.tier == 2 ? 768u : 1024u;
         const uint64_t sparse = base + 4u + 64u * (h.keep + 1u);
-        const uint64_t coords = sparse + 4u + 64u + 32u;
-        uint64_t e = coords;
-        if (h.count > 0) {
-            e = coords + 2u * (uint64_t)(h.count > cap ? h.count : cap);
-            const uint64_t nib = coords + 2u * cap + (h.count + 1u) / 2u;
-            if (nib > e) e = nib;
+        const uint64_t binsBase = sparse + 68u;
+        if (binsBase + 32u > a.payloadBytes) return binsBase + 32u;
+        // the binary search visits indices below max(bins[], count)
+        uint32_t maxIdx = h.count;
+        for (uint32_t i = 0; i < 16u; ++i) {
+            const uint32_t v = b.u16((uint32_t)(binsBase + 2u * i));
+            maxIdx = v > maxIdx ? v : maxIdx;
         }
-        if (e > end) end = e;
-    } else if (h.mode >= 2u) {
+        const uint64_t coordsEnd = binsBase + 32u + 2u * (uint64_t)maxIdx;
+        const uint64_t nibEnd = binsBase + 32u + 2u * (uint64_t)cap + ((uint64_t)maxIdx + 1u) / 2u;
+        end = coordsEnd > nibEnd ? coordsEnd : nibEnd;
+        if (binsBase + 32u > end) end = binsBase + 32u;
+    } else if (h.mode >= 2u && !(h.subtype == 1u && (h.tk == 0u || h.bits == 0u))) {
         const uint32_t R = h.mode == 2u ? 3u : 4u;
         const uint64_t vals = (uint64_t)R * R * R;
         const uint64_t blockBytes = (vals * h.bits + 7u) / 8u;
         const uint64_t scales = base + 4u + 64u * (h.keep + 1u);
         const uint64_t nb = h.subtype == 1u ? h.tk : a.F;
-        const uint64_t e = scales + 2u * nb + nb * blockBytes + 1u; // BFP reads may touch one byte past a block
-        if (h.subtype != 1u || (h.tk > 0 && h.bits > 0))
-            if (e > end) end = e;
+        const uint64_t lastBit = (vals - 1u) * h.bits;
+        const uint64_t lastByte = lastBit / 8u + ((lastBit % 8u) + h.bits > 8u ? 2u : 1u);
+        end = scales + 2u * nb + (nb - 1u) * blockBytes + lastByte;
+        if (scales + 2u * nb > end) end = scales + 2u * nb;
     }
     return end;
 }
