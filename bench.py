@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the VBTPACK4 random-access decode path on one MI355X.
+
+Workload (default): the single-field frame-cache build -- synthetic density
+256x256x256 over 120 frames (24 advected Gaussian blobs, sigma 3-16, ~69%
+empty leaves), VBTPACK4-encoded by this repository's encoder, resident in
+HBM.  One step = decode every frame (t = 0..119) of every active leaf into the
+packed compact active-leaf cache (meaningful controls only).  Also reported:
+point-query throughput/latency of the batched kernel vs the plain
+one-request-per-launch GPU decode, and the CPU oracle's decode rate on the
+host cores (bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(info, nt: int) -> dict:
+    """Frame-cache build bytes per launch: compressed payload + offset table
+    read once, nt frames of meaningful control floats written, page table."""
+    L = info.header.leafCount
+    written = 4.0 * info.controlsPerFrame * nt
+    read = float(info.payloadBytes) + 4.0 * (L + 1)
+    return {"read": read, "written": written, "page_table": 4.0 * L, "total": read + written + 4.0 * L}
+
+
+def cpu_baseline(data: bytes, frames: int, budget_s: float, threads: int) -> dict:
+    """Oracle (C restatement of the reference decode, -O2) on the host cores:
+    packed frame-cache decode of whole frames, one frame per thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+
+    import ctypes
+
+    O = oracle_py.oracle()
+    of = oracle_py.OracleFile(data)
+    _, offs = of.cache_packed(0)  # the packed-cache offsets (computed once)
+    per_frame_bytes = 4.0 * float(offs[-1])
+
+    def decode(f, t, out):
+        rc = O.vbto_frame_cache_packed(ctypes.byref(f.vf), t, offs.ctypes.data, out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(oracle_py.last_error())
+
+    # one thread first (the reference's serial loop)
+    buf = np.empty(int(offs[-1]), np.float32)
+    t0 = time.perf_counter()
+    n1 = 0
+    while time.perf_counter() - t0 < budget_s / 3:
+        decode(of, n1 % frames, buf)
+        n1 += 1
+    dt1 = time.perf_counter() - t0
+    # all threads (independent frames; ctypes releases the GIL)
+    counter = {"n": 0}
+    lock = threading.Lock()
+    stop = time.perf_counter() + budget_s * 2 / 3
+
+    def work(tid):
+        f = oracle_py.OracleFile(data)
+        b = np.empty(int(offs[-1]), np.float32)
+        t = tid
+        while time.perf_counter() < stop:
+            decode(f, t % frames, b)
+            with lock:
+                counter["n"] += 1
+            t += threads
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    t0 = time.perf_counter()
+    for th in ts:
+        th.start()
+    for th in ts:
+        th.join()
+    dtn = time.perf_counter() - t0
+    return {
+        "value": round(counter["n"] * per_frame_bytes / dtn / 1e9, 4),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{counter['n']} whole-frame cache decodes ({dtn:.1f} s, {threads} threads) of the same asset; "
+                  f"1 thread: {n1} frames in {dt1:.1f} s = {n1 * per_frame_bytes / dt1 / 1e9:.4f} GB/s",
+        "single_thread_gbs": round(n1 * per_frame_bytes / dt1 / 1e9, 4),
+    }
+
+
+def query_bench(asset, hip, n_batch: int, reps: int) -> dict:
+    """Batched point queries (one launch over the batch, 8 lanes per query)
+    vs the plain GPU decode (one thread per query); latency per batch."""
+    W, H, D, F = asset.dims
+    rng = np.random.default_rng(0x5EED0001)
+    q = np.stack([rng.integers(0, W, n_batch), rng.integers(0, H, n_batch), rng.integers(0, D, n_batch),
+                  rng.integers(0, F, n_batch)], axis=1).astype(np.uint32)
+    s = asset.stream
+    dq = hip.DeviceArray.from_host(q, s)
+    out = hip.DeviceArray(n_batch, np.float32, s)
+    res = {}
+    for name, plain in (("batched", False), ("plain", True)):
+        for _ in range(3):
+            asset.query_into(dq, n_batch, out, plain=plain)
+        s.sync()
+        ev0, ev1 = hip.Event(), hip.Event()
+        lat = []
+        for _ in range(reps):
+            ev0.record(s)
+            asset.query_into(dq, n_batch, out, plain=plain)
+            ev1.record(s)
+            lat.append(ev0.elapsed_ms(ev1) * 1e3)
+        lat = np.array(lat)
+        res[name] = {"qps": round(n_batch / (np.median(lat) * 1e-6), 1), "p50_us": round(float(np.percentile(lat, 50)), 2),
+                     "p99_us": round(float(np.percentile(lat, 99)), 2)}
+    res["batch"] = n_batch
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="smoke256")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the decode shards by leaf or "
+              "frame ranges with replicated assets; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from vbt_amd import hip, synth
+
+    if hip.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    w = synth.WORKLOADS[args.workload]
+    t0 = time.time()
+    data = w.encode(threads=min(16, os.cpu_count() or 1))
+    enc_s = time.time() - t0
+    print(f"[bench] encoded {args.workload} {w.dims}: {len(data) / 1e6:.1f} MB in {enc_s:.1f} s", file=sys.stderr,
+          flush=True)
+    s = hip.Stream()
+    asset = hip.Asset(data, s)
+    info = asset.info
+    W, H, D, F = asset.dims
+    nt = F
+    out = hip.DeviceArray((nt, info.controlsPerFrame), np.float32, s)
+    for _ in range(args.warmup):
+        asset.frame_cache_into(out, 0, nt)
+    s.sync()
+    ev0, ev1 = hip.Event(), hip.Event()
+    per = []
+    t_wall = time.perf_counter()
+    for _ in range(args.steps):
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(s)
+        asset.frame_cache_into(out, 0, nt)
+        e1.record(s)
+        per.append((e0, e1))
+    s.sync()
+    wall = time.perf_counter() - t_wall
+    ms = np.array([a.elapsed_ms(b) for a, b in per])
+    ms_step = float(ms.mean())
+    byt = algorithmic_bytes(info, nt)
+    decoded_gbs = byt["written"] / (ms_step * 1e-3) / 1e9
+    achieved = byt["total"] / (ms_step * 1e-3) / 1e9
+    print(f"[bench] {args.steps} steps: kernel mean {ms_step:.4f} ms (min {ms.min():.4f}), wall/step "
+          f"{wall / args.steps * 1e3:.4f} ms", file=sys.stderr, flush=True)
+
+    qres = query_bench(asset, hip, 65536, 50)
+    small = query_bench(asset, hip, 256, 200)
+
+    line = {
+        "metric": "decoded GB/s + % HBM roofline; batched query p50/p99 us vs plain GPU decode",
+        "value": round(decoded_gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: frame-cache build {W}x{H}x{D} x {F} frames, all frames per step",
+                   "active_leaves": info.activeLeaves, "leaves": info.header.leafCount,
+                   "controls_per_frame": info.controlsPerFrame, "payload_mb": round(info.payloadBytes / 1e6, 2),
+                   "fast_leaves": info.fast_leaves},
+        "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": byt["total"], "kernel": "k_cache_render<0>"},
+        "queries": {"random_65536": qres, "random_256": small},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(data, F, args.cpu_seconds, min(16, os.cpu_count() or 1))
+    asset.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

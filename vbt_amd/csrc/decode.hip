@@ -370,19 +370,63 @@ __device__ inline uint32_t find_leaf(const uint64_t* ctrl, uint32_t A, uint64_t 
 }
 
 // ---------------------------------------------------------- cache: render --
+// Lerp weight of the sweep: alpha = (frame - f0) / (f1 - f0) for integer
+// frames 0 <= frame - f0 < f1 - f0 < 2^16.  With r = RN64(1 / (f1 - f0)),
+// RN32(RN64((frame - f0) * r)) equals the correctly rounded float quotient the
+// reference computes: the double product is within 2^-52 (relative) of the
+// true quotient, while a quotient a/b with b < 2^16 lies at least 2^-41
+// (relative) away from every float rounding boundary, so both round alike.
+__device__ inline float sweep_alpha(int num, double rinv) { return (float)((double)num * rinv); }
+
 // One thread per (active leaf, cache slot); the thread produces that control
-// for frames t0..t0+nt-1.  Leaves whose every control has strictly increasing
-// keyframe frames use a forward sweep over the keyframes (each frame's bracket
-// is the adjacent key pair around it, which is exactly the bracket the
-// reference's hinted binary search returns for increasing keys); other leaves
-// run the reference bracket search per frame.
+// for frames t0..t0+nt-1.  The workgroup first locates its leaf range in the
+// page table (two binary searches per workgroup) and stages the leaves'
+// payload bytes -- contiguous in the pool because active leaves are in leaf
+// order -- into LDS with dword loads, so keyframe reads hit LDS.  Leaves
+// whose every control has strictly increasing keyframe frames use a forward
+// sweep over the keyframes: each frame's bracket is then the adjacent key pair
+// around it, which is exactly the bracket the reference's hinted binary
+// search returns for increasing keys.  Other leaves run the reference
+// bracket search per frame.
+constexpr uint32_t CACHE_LDS_BYTES = 32768;
+
 template <int LAYOUT>
 __global__ void __launch_bounds__(256) k_cache_render(AssetDev a, int t0, int nt, float* out, uint64_t total,
                                                       uint32_t* errFlag)
 {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t lds[CACHE_LDS_BYTES / 4];
+    __shared__ uint32_t sA0, sA1, sWord0, sStaged;
+    const uint64_t gBase = (uint64_t)blockIdx.x * blockDim.x;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) sA0 = find_leaf(a.ctrl, a.A, gBase);
+    if (tid == 64) {
+        const uint64_t gl = gBase + blockDim.x - 1u < total ? gBase + blockDim.x - 1u : total - 1u;
+        sA1 = find_leaf(a.ctrl, a.A, gl);
+    }
+    __syncthreads();
+    const uint32_t a0 = sA0, a1 = sA1;
+    if (tid == 0) {
+        const uint32_t w0 = a.leaves[a0].word;
+        const uint32_t w1 = a.leaves[a1].word + a.leaves[a1].bytes / 4u;
+        sWord0 = w0;
+        sStaged = (w1 - w0) * 4u <= CACHE_LDS_BYTES ? (w1 - w0) : 0u;
+    }
+    __syncthreads();
+    const uint32_t word0 = sWord0, nWords = sStaged;
+    const uint32_t* pw = (const uint32_t*)a.payload;
+    for (uint32_t i = tid; i < nWords; i += blockDim.x) lds[i] = pw[word0 + i];
+    __syncthreads();
+
+    const uint64_t g = gBase + tid;
     if (g >= total) return;
-    const uint32_t c = find_leaf(a.ctrl, a.A, g);
+    // leaf of this slot: binary search inside the workgroup's small range
+    uint32_t lo = a0, hi = a1 + 1u;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.ctrl[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t c = lo;
     const DevLeaf& d = a.leaves[c];
     const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
     const uint32_t status = d.status;
@@ -406,7 +450,7 @@ __global__ void __launch_bounds__(256) k_cache_render(AssetDev a, int t0, int nt
     const bool isCoarse = slot < d.cc;
     const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
     const uint32_t ci = isCoarse ? slot : slot - d.cc;
-    const Bytes b{a.payload + (size_t)d.word * 4u};
+    const Bytes b{nWords ? (const uint8_t*)lds + (size_t)(d.word - word0) * 4u : a.payload + (size_t)d.word * 4u};
     const int F = (int)a.F;
     const uint32_t dsc = s.base + ci * 4u;
     const uint32_t keyStart = b.u16(dsc), keyCount = b.u8(dsc + 2u);
@@ -428,6 +472,7 @@ __global__ void __launch_bounds__(256) k_cache_render(AssetDev a, int t0, int nt
     float vk = key_value(b, s.keyBase, s.codec, keyStart);
     int fn = last > 0u ? key_frame(b, s.keyBase, s.codec, keyStart + 1u) : 0;
     float vn = last > 0u ? key_value(b, s.keyBase, s.codec, keyStart + 1u) : 0.0f;
+    double rinv = last > 0u ? 1.0 / (double)(fn - fk) : 0.0;
     for (int i = 0; i < nt; ++i) {
         const int frame = clamp_frame(t0 + i, F);
         while (k < last && fn <= frame) {
@@ -437,13 +482,14 @@ __global__ void __launch_bounds__(256) k_cache_render(AssetDev a, int t0, int nt
             if (k < last) {
                 fn = key_frame(b, s.keyBase, s.codec, keyStart + k + 1u);
                 vn = key_value(b, s.keyBase, s.codec, keyStart + k + 1u);
+                rinv = 1.0 / (double)(fn - fk);
             }
         }
         float v;
         if (fk >= frame || k == last) {
             v = vk;
         } else {
-            const float alpha = (float)(frame - fk) / (float)(fn - fk);
+            const float alpha = sweep_alpha(frame - fk, rinv);
             v = vk + alpha * (vn - vk);
         }
         out[(uint64_t)i * stride] = v;
@@ -1186,7 +1232,7 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     if (a->h.profileType == 2 && layout != VBT_CACHE_PACKED)
         return vbt_fail(VBT_ERR_INVALID, "scientific assets support the packed cache only");
     hipStream_t s = S(stream);
-    k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
+    if (a->bad) k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
     const AssetDev ad = a->dev();
     if (a->A == 0) return VBT_OK;
     if (a->h.profileType == 2) {
@@ -1211,7 +1257,7 @@ int vbt_decode_frame_dense(vbt_asset a, int t, const vbt_compose* c, float* d_ou
     if (a->h.profileType == 2 && (t < 0 || (uint32_t)t >= a->h.frames))
         return vbt_fail(VBT_ERR_INVALID, "scientific frame index out of range");
     hipStream_t s = S(stream);
-    k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
+    if (a->bad) k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
     const AssetDev ad = a->dev();
     if (a->h.profileType == 2) k_sci_leaf<true><<<dim3(a->A, 1), 512, 0, s>>>(ad, t, 1, d_out, 0, a->d_err);
     else k_dense_render<<<a->h.leafCount, 512, 0, s>>>(ad, t, to_dev(c), d_out, a->d_err);
@@ -1226,7 +1272,6 @@ static int launch_query(vbt_asset a, const uint32_t* d_q, uint32_t n, const vbt_
     if (!a || !d_q || !d_out) return vbt_fail(VBT_ERR_INVALID, "invalid query request");
     if (n == 0) return VBT_OK;
     hipStream_t s = S(stream);
-    k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
     const AssetDev ad = a->dev();
     if (plain) k_query<1><<<blocks_for(n, 256), 256, 0, s>>>(ad, (const uint4*)d_q, n, to_dev(c), d_out, d_status, a->d_err);
     else k_query<8><<<blocks_for(8ull * n, 256), 256, 0, s>>>(ad, (const uint4*)d_q, n, to_dev(c), d_out, d_status, a->d_err);

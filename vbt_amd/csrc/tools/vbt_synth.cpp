@@ -833,7 +833,7 @@ int encode_scientific(const Field& fd, const vbts_encode_opts& o, std::vector<ui
             }
             tail.insert(tail.end(), nib.begin(), nib.end());
             header = 1u | ((uint32_t)(keep - 1) << 2) | (tier << 8) | ((uint32_t)(fpb - 1) << 10) | (n << 16);
-        } else if (rms > 0.02 * fd.amp) {
+        } else if (rms > 0.004 * fd.amp) {
             const int variant = (int)(lh64 % 4u);
             const int R = (variant & 1) ? 3 : 4;
             const int sub = (variant >> 1) & 1;
@@ -987,7 +987,7 @@ void vbts_default_encode_opts(vbts_encode_opts* o)
     o->codec = 0;
     o->epsAbs = 1e-4f;
     o->epsRel = 0.02f;
-    o->fineThreshold = 0.05f;
+    o->fineThreshold = 0.02f;
     o->shellBand = 2.0f;
     o->burstFraction = 0.0f;
     o->burstMax = 32;
