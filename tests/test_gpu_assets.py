@@ -128,3 +128,39 @@ def test_corrupt_leaf_fails_alone():
     good = oracle_py.OracleFile(SMALL["levelset"].encode(threads=8))
     bitexact(vals[1:], good.query(q[1:]), "queries beside a poisoned leaf")
     a.close()
+
+
+def all_halves_asset():
+    """5 Grid4 leaves whose 64 controls hold a keyframe at every frame 0..254
+    with raw half bit patterns covering all 65536 values (zeros, subnormals,
+    normals, infinities, NaN payloads): at frame f the cache holds key f."""
+    F = 255
+    leaves = []
+    for leaf in range(5):
+        desc = b"".join(struct.pack("<HBB", c * F, F, 32) for c in range(64))
+        bins = b"".join(bytes([min(255, (b * F // 8))]) for _ in range(64) for b in range(8))
+        keys = bytearray()
+        for c in range(64):
+            for j in range(F):
+                h = (leaf * 64 * F + c * F + j) % 65536
+                keys += struct.pack("<BH", j, h)
+        keys += b"\0" * ((-len(keys)) % 4)
+        hdr = (1 | (0 << 2) | (4 << 6))
+        leaves.append(struct.pack("<IHH", hdr, 64 * F, 0) + desc + bins + bytes(keys))
+    offs = [0]
+    for lb in leaves:
+        offs.append(offs[-1] + len(lb) // 4)
+    return (b"VBTPACK4" + struct.pack("<11I", 4, 40, 8, 8, F, 8, 4, 0, 5, 1, 0) +
+            struct.pack(f"<{len(offs)}I", *offs) + b"".join(leaves))
+
+
+def test_every_half_pattern_through_the_sweep():
+    data = all_halves_asset()
+    o = oracle_py.OracleFile(data)
+    a = hip.Asset(data)
+    assert a.info.fast_leaves == 5
+    got = a.frame_cache(0, 255)
+    for t in range(0, 255, 1):
+        exp, _ = o.cache_packed(t)
+        np.testing.assert_array_equal(got[t].view(np.uint32), exp.view(np.uint32), err_msg=f"t={t}")
+    a.close()

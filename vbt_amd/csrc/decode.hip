@@ -388,102 +388,64 @@ __device__ inline float sweep_alpha(int num, double rinv) { return (float)((doub
 // around it, which is exactly the bracket the reference's hinted binary
 // search returns for increasing keys.  Other leaves run the reference
 // bracket search per frame.
-constexpr uint32_t CACHE_LDS_BYTES = 32768;
 
-template <int LAYOUT>
-__global__ void __launch_bounds__(256) k_cache_render(AssetDev a, int t0, int nt, float* out, uint64_t total,
-                                                      uint32_t* errFlag)
+// Branch-free form of half_to_float for the sweep: the hardware conversion is
+// exact for zeros, subnormals, normals and infinities; NaNs take the
+// reference's bit mapping (payload shifted, no quieting).
+__device__ __forceinline__ float sweep_half(uint32_t h)
 {
-    __shared__ uint32_t lds[CACHE_LDS_BYTES / 4];
-    __shared__ uint32_t sA0, sA1, sWord0, sStaged;
-    const uint64_t gBase = (uint64_t)blockIdx.x * blockDim.x;
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) sA0 = find_leaf(a.ctrl, a.A, gBase);
-    if (tid == 64) {
-        const uint64_t gl = gBase + blockDim.x - 1u < total ? gBase + blockDim.x - 1u : total - 1u;
-        sA1 = find_leaf(a.ctrl, a.A, gl);
-    }
-    __syncthreads();
-    const uint32_t a0 = sA0, a1 = sA1;
-    if (tid == 0) {
-        const uint32_t w0 = a.leaves[a0].word;
-        const uint32_t w1 = a.leaves[a1].word + a.leaves[a1].bytes / 4u;
-        sWord0 = w0;
-        sStaged = (w1 - w0) * 4u <= CACHE_LDS_BYTES ? (w1 - w0) : 0u;
-    }
-    __syncthreads();
-    const uint32_t word0 = sWord0, nWords = sStaged;
-    const uint32_t* pw = (const uint32_t*)a.payload;
-    for (uint32_t i = tid; i < nWords; i += blockDim.x) lds[i] = pw[word0 + i];
-    __syncthreads();
+    _Float16 x;
+    const uint16_t hb = (uint16_t)h;
+    __builtin_memcpy(&x, &hb, 2);
+    const float f = (float)x;
+    const uint32_t nanBits = ((h & 0x8000u) << 16) | 0x7F800000u | ((h & 0x3FFu) << 13);
+    return ((h & 0x7C00u) == 0x7C00u && (h & 0x3FFu)) ? __uint_as_float(nanBits) : f;
+}
 
-    const uint64_t g = gBase + tid;
-    if (g >= total) return;
-    // leaf of this slot: binary search inside the workgroup's small range
-    uint32_t lo = a0, hi = a1 + 1u;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.ctrl[mid] <= g) lo = mid;
-        else hi = mid;
-    }
-    const uint32_t c = lo;
-    const DevLeaf& d = a.leaves[c];
-    const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
-    const uint32_t status = d.status;
-    uint64_t stride;
-    uint64_t dst;
-    if (LAYOUT == VBT_CACHE_PACKED) {
-        stride = total;
-        dst = g;
+template <uint32_t KB>
+__device__ __forceinline__ void sweep_key(const uint8_t* p, uint32_t i, int& f, float& v)
+{
+    if (KB == 8u) {
+        const uint8_t* q = p + i * 8u;
+        f = (int)((uint32_t)q[0] | ((uint32_t)q[1] << 8));
+        v = __uint_as_float((uint32_t)q[4] | ((uint32_t)q[5] << 8) | ((uint32_t)q[6] << 16) | ((uint32_t)q[7] << 24));
     } else {
-        const uint32_t s576 = slot < d.cc ? slot : 64u + (slot - d.cc);
-        if ((slot < d.cc && slot >= 64u) || s576 >= 576u) return;
-        stride = (uint64_t)a.A * 576u;
-        dst = (uint64_t)c * 576u + s576;
+        const uint8_t* q = p + i * 3u;
+        f = (int)q[0];
+        v = sweep_half((uint32_t)q[1] | ((uint32_t)q[2] << 8));
     }
-    out += dst;
-    if ((status & STATUS_MASK) != LEAF_OK) {
-        for (int i = 0; i < nt; ++i) out[(uint64_t)i * stride] = __builtin_nanf("");
-        atomicOr(errFlag, 1u);
-        return;
-    }
-    const bool isCoarse = slot < d.cc;
-    const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
-    const uint32_t ci = isCoarse ? slot : slot - d.cc;
-    const Bytes b{nWords ? (const uint8_t*)lds + (size_t)(d.word - word0) * 4u : a.payload + (size_t)d.word * 4u};
-    const int F = (int)a.F;
-    const uint32_t dsc = s.base + ci * 4u;
-    const uint32_t keyStart = b.u16(dsc), keyCount = b.u8(dsc + 2u);
-    if (keyCount == 0u) {
-        for (int i = 0; i < nt; ++i) out[(uint64_t)i * stride] = 0.0f;
-        return;
-    }
-    if (!(status & FLAG_SWEEP)) {
-        const uint32_t binBase = s.base + s.count * 4u + ci * s.bins;
-        for (int i = 0; i < nt; ++i)
-            out[(uint64_t)i * stride] =
-                decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins, F, t0 + i);
-        return;
-    }
-    // sweep: k = last key with frame <= current frame (or 0)
+}
+
+// Sweep of one control over frames t0..t0+nt-1 (out already offset to the
+// slot; consecutive frames are `stride` floats apart).  `keys` points at the
+// control's first keyframe record (LDS or global; the caller instantiates
+// both so that the LDS path compiles to ds_read and the loop never waits for
+// its own outstanding global stores).  Requires strictly increasing frames.
+// The key after next is prefetched at every advance so its LDS latency
+// overlaps the following frames; `rtab[b]` = RN64(1/b) for gaps b < 256.
+template <uint32_t KB>
+__device__ __forceinline__ void sweep_slot(const uint8_t* keys, uint32_t keyCount, int F, int t0, int nt, float* out,
+                                           uint64_t stride, const double* rtab)
+{
     const uint32_t last = keyCount - 1u;
     uint32_t k = 0;
-    int fk = key_frame(b, s.keyBase, s.codec, keyStart);
-    float vk = key_value(b, s.keyBase, s.codec, keyStart);
-    int fn = last > 0u ? key_frame(b, s.keyBase, s.codec, keyStart + 1u) : 0;
-    float vn = last > 0u ? key_value(b, s.keyBase, s.codec, keyStart + 1u) : 0.0f;
-    double rinv = last > 0u ? 1.0 / (double)(fn - fk) : 0.0;
+    int fk, fn = 0, fp = 0;
+    float vk, vn = 0.0f, vp = 0.0f;
+    sweep_key<KB>(keys, 0, fk, vk);
+    if (last >= 1u) sweep_key<KB>(keys, 1, fn, vn);
+    if (last >= 2u) sweep_key<KB>(keys, 2, fp, vp);
+    auto recip = [&](int gap) { return gap < 256 ? rtab[gap] : 1.0 / (double)gap; };
+    double rinv = last > 0u ? recip(fn - fk) : 0.0;
     for (int i = 0; i < nt; ++i) {
         const int frame = clamp_frame(t0 + i, F);
         while (k < last && fn <= frame) {
             ++k;
             fk = fn;
             vk = vn;
-            if (k < last) {
-                fn = key_frame(b, s.keyBase, s.codec, keyStart + k + 1u);
-                vn = key_value(b, s.keyBase, s.codec, keyStart + k + 1u);
-                rinv = 1.0 / (double)(fn - fk);
-            }
+            fn = fp;
+            vn = vp;
+            if (k < last) rinv = recip(fn - fk);
+            if (k + 2u <= last) sweep_key<KB>(keys, k + 2u, fp, vp);
         }
         float v;
         if (fk >= frame || k == last) {
@@ -493,6 +455,143 @@ __global__ void __launch_bounds__(256) k_cache_render(AssetDev a, int t0, int nt
             v = vk + alpha * (vn - vk);
         }
         out[(uint64_t)i * stride] = v;
+    }
+}
+
+// Workgroup size of the cache kernel (the block -> first-leaf table is built
+// for it at asset open) and its LDS keyframe window.
+constexpr uint32_t CACHE_BLOCK = 128;
+constexpr uint32_t CACHE_KEY_LDS = 16384;
+constexpr uint32_t CACHE_WAVES = CACHE_BLOCK / 64;
+
+// first active leaf of every CACHE_BLOCK-slot tile of the packed cache
+__global__ void k_block_leaf(const uint64_t* ctrl, uint32_t A, uint64_t total, uint32_t nb, uint32_t* blockLeaf)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    const uint64_t g = (uint64_t)b * CACHE_BLOCK;
+    blockLeaf[b] = g < total ? find_leaf(ctrl, A, g) : (A ? A - 1u : 0u);
+}
+
+__device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t* sm, uint32_t lane, uint32_t wave)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, 64));
+    if (lane == 0) sm[wave] = v;
+    __syncthreads();
+    v = sm[0];
+#pragma unroll
+    for (uint32_t w = 1; w < CACHE_WAVES; ++w) v = min(v, sm[w]);
+    __syncthreads();
+    return v;
+}
+
+__device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* sm, uint32_t lane, uint32_t wave)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    if (lane == 0) sm[wave] = v;
+    __syncthreads();
+    v = sm[0];
+#pragma unroll
+    for (uint32_t w = 1; w < CACHE_WAVES; ++w) v = max(v, sm[w]);
+    __syncthreads();
+    return v;
+}
+
+// One thread per (active leaf, cache slot): the thread produces that control
+// for frames t0..t0+nt-1.  The workgroup reads its first leaf from the block
+// table and each thread locates its leaf and stream.  Keyframe records of the
+// workgroup's controls are contiguous in the pool (consecutive controls of a
+// stream store consecutive key ranges; active leaves are in leaf order), so
+// the workgroup stages them into an LDS window with dword loads, in as many
+// passes as the window needs: each pass serves the controls whose records
+// lie inside it.  Leaves whose every control has strictly increasing
+// keyframe frames use the forward sweep (each frame's bracket is then the
+// adjacent key pair around it, exactly the bracket the reference's hinted
+// binary search returns for increasing keys); other leaves run the
+// reference bracket search per frame from global memory.
+template <int LAYOUT>
+__global__ void __launch_bounds__(CACHE_BLOCK) k_cache_render(AssetDev a, const uint32_t* blockLeaf, int t0, int nt,
+                                                              float* out, uint64_t total, uint32_t* errFlag)
+{
+    __shared__ uint32_t lds[CACHE_KEY_LDS / 4];
+    __shared__ uint32_t sRed[CACHE_WAVES];
+    __shared__ double rtab[256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t i = tid; i < 256u; i += CACHE_BLOCK) rtab[i] = i ? 1.0 / (double)i : 0.0;
+    const uint64_t g = (uint64_t)blockIdx.x * CACHE_BLOCK + tid;
+    const bool live = g < total;
+    uint32_t c = blockLeaf[blockIdx.x];
+    if (live)
+        while (c + 1u < a.A && a.ctrl[c + 1u] <= g) ++c;
+    const DevLeaf& d = a.leaves[c];
+    const uint32_t slot = live ? (uint32_t)(g - a.ctrl[c]) : 0u;
+    const uint32_t status = d.status;
+    const bool isCoarse = slot < d.cc;
+    const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
+    const uint32_t ci = isCoarse ? slot : slot - d.cc;
+    const uint32_t leafByte = d.word * 4u;
+    uint32_t keyStart = 0, keyCount = 0;
+    const bool ok = live && (status & STATUS_MASK) == LEAF_OK;
+    if (ok) {
+        const uint32_t desc = *(const uint32_t*)(a.payload + leafByte + s.base + ci * 4u); // descriptors are 4-aligned
+        keyStart = desc & 0xFFFFu;
+        keyCount = (desc >> 16) & 0xFFu;
+    }
+    const uint32_t kb = keyframe_bytes(s.codec);
+    const bool sweep = ok && keyCount > 0u && (status & FLAG_SWEEP);
+    const uint32_t kLo = leafByte + s.keyBase + keyStart * kb;
+    const uint32_t kHi = kLo + keyCount * kb;
+
+    uint64_t stride = 0, dst = 0;
+    bool writes = live;
+    if (LAYOUT == VBT_CACHE_PACKED) {
+        stride = total;
+        dst = g;
+    } else {
+        const uint32_t s576 = isCoarse ? slot : 64u + (slot - d.cc);
+        if ((isCoarse && slot >= 64u) || s576 >= 576u) writes = false;
+        stride = (uint64_t)a.A * 576u;
+        dst = (uint64_t)c * 576u + s576;
+    }
+    float* o = out + dst;
+    const int F = (int)a.F;
+    if (writes && !sweep) {
+        if (!ok) {
+            for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = __builtin_nanf("");
+            atomicOr(errFlag, 1u);
+        } else if (keyCount == 0u) {
+            for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = 0.0f;
+        } else {
+            const Bytes b{a.payload + leafByte};
+            const uint32_t binBase = s.base + s.count * 4u + ci * s.bins;
+            for (int i = 0; i < nt; ++i)
+                o[(uint64_t)i * stride] =
+                    decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins, F, t0 + i);
+        }
+    }
+    bool pending = sweep && writes;
+    for (;;) {
+        // window start: the lowest pending record; the window serves every
+        // pending control whose records end inside it
+        const uint32_t lo = block_min(pending ? kLo : 0xFFFFFFFFu, sRed, lane, wave);
+        if (lo == 0xFFFFFFFFu) break;
+        const uint32_t w0 = lo / 4u;
+        const uint32_t limit = w0 * 4u + CACHE_KEY_LDS;
+        const bool mine = pending && kHi <= limit;
+        const uint32_t hi = block_max(mine ? kHi : 0u, sRed, lane, wave);
+        const uint32_t nw = (hi + 3u) / 4u - w0;
+        const uint32_t* pw = (const uint32_t*)a.payload;
+        for (uint32_t i = tid; i < nw; i += CACHE_BLOCK) lds[i] = pw[w0 + i];
+        __syncthreads();
+        if (mine) {
+            const uint8_t* p = (const uint8_t*)lds + (kLo - w0 * 4u);
+            if (kb == 8u) sweep_slot<8u>(p, keyCount, F, t0, nt, o, stride, rtab);
+            else sweep_slot<3u>(p, keyCount, F, t0, nt, o, stride, rtab);
+            pending = false;
+        }
+        __syncthreads();
     }
 }
 
@@ -1000,10 +1099,12 @@ struct vbt_asset_s {
     uint64_t* d_ctrl = nullptr;
     float* d_basis = nullptr;
     uint32_t* d_err = nullptr;   // error flag word for launches
+    uint32_t* d_blockLeaf = nullptr; // first active leaf of each cache-kernel tile
     Pair* d_blocks = nullptr;
     uint32_t A = 0, bad = 0;
     uint64_t controlsPerFrame = 0, payloadBytes = 0, deviceBytes = 0, fast = 0;
     uint64_t modeCounts[4] = {0, 0, 0, 0};
+    std::string badMessage; // first poisoned leaf and its reference exception text
 
     AssetDev dev() const
     {
@@ -1041,20 +1142,20 @@ template <class T> int dalloc(vbt_asset a, T** p, size_t count)
 void free_asset(vbt_asset a)
 {
     if (!a) return;
-    void* ptrs[] = {a->d_offsets, a->d_payload, a->d_scales, a->d_compact, a->d_status,
-                    a->d_leaves,  a->d_ctrl,    a->d_basis,  a->d_err,     a->d_blocks};
+    void* ptrs[] = {a->d_offsets, a->d_payload, a->d_scales, a->d_compact, a->d_status,     a->d_leaves,
+                    a->d_ctrl,    a->d_basis,   a->d_err,    a->d_blocks,  a->d_blockLeaf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete a;
 }
 
-int check_err_flag(vbt_asset a, hipStream_t s, const char* what)
+// Whole-frame requests decode every leaf, so an asset with poisoned leaves
+// fails them (as the reference's decode throws); the valid leaves' outputs are
+// still written and the poisoned ones are NaN or absent.
+int report_bad(vbt_asset a, const char* what)
 {
-    uint32_t flag = 0;
-    HIPCHK(hipMemcpyAsync(&flag, a->d_err, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (flag) return vbt_fail(VBT_ERR_FORMAT, std::string(what) + ": request touches a leaf that failed validation");
-    return VBT_OK;
+    return vbt_fail(VBT_ERR_FORMAT, std::string(what) + ": " + std::to_string(a->bad) +
+                                        " leaves failed validation; " + a->badMessage);
 }
 
 ComposeDev to_dev(const vbt_compose* c)
@@ -1157,6 +1258,13 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
     TRY(hipMemcpyAsync(a->d_ctrl + a->A, &a->controlsPerFrame, 8, hipMemcpyHostToDevice, s));
     if (a->A) k_validate<<<a->A, 256, 0, s>>>(ad, a->d_leaves, a->d_status);
     TRY(hipGetLastError());
+    {
+        const uint32_t nbc = blocks_for(a->controlsPerFrame, CACHE_BLOCK);
+        if ((rc = dalloc(a, &a->d_blockLeaf, (size_t)nbc + 1))) return fail(rc);
+        k_block_leaf<<<blocks_for(nbc + 1ull, 256), 256, 0, s>>>(a->d_ctrl, a->A, a->controlsPerFrame, nbc,
+                                                                  a->d_blockLeaf);
+        TRY(hipGetLastError());
+    }
     // host-side statistics of the page table
     std::vector<uint32_t> st(L);
     TRY(hipMemcpyAsync(st.data(), a->d_status, 4ull * L, hipMemcpyDeviceToHost, s));
@@ -1167,7 +1275,11 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
         uint32_t hw;
         std::memcpy(&hw, base + payOff + 4ull * w, 4);
         a->modeCounts[hw & 3u]++;
-        if ((st[l] & STATUS_MASK) != LEAF_OK) a->bad++;
+        if ((st[l] & STATUS_MASK) != LEAF_OK) {
+            if (a->bad == 0)
+                a->badMessage = "first: leaf " + std::to_string(l) + ": " + vbt_status_message((int32_t)(st[l] & STATUS_MASK));
+            a->bad++;
+        }
         if (st[l] & FLAG_SWEEP) a->fast++;
     }
 #undef TRY
@@ -1232,22 +1344,21 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     if (a->h.profileType == 2 && layout != VBT_CACHE_PACKED)
         return vbt_fail(VBT_ERR_INVALID, "scientific assets support the packed cache only");
     hipStream_t s = S(stream);
-    if (a->bad) k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
     const AssetDev ad = a->dev();
-    if (a->A == 0) return VBT_OK;
+    if (a->A == 0) return a->bad ? report_bad(a, "frame cache") : VBT_OK;
     if (a->h.profileType == 2) {
         k_sci_leaf<false><<<dim3(a->A, (uint32_t)nt), 512, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
     } else if (layout == VBT_CACHE_PACKED) {
-        k_cache_render<VBT_CACHE_PACKED>
-            <<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        k_cache_render<VBT_CACHE_PACKED><<<blocks_for(a->controlsPerFrame, CACHE_BLOCK), CACHE_BLOCK, 0, s>>>(
+            ad, a->d_blockLeaf, t0, nt, d_out, a->controlsPerFrame, a->d_err);
     } else {
         const uint64_t n = (uint64_t)a->A * 576u * (uint64_t)nt;
         k_fill<<<(uint32_t)std::min<uint64_t>(blocks_for(n, 256), 65535), 256, 0, s>>>(d_out, n, 0.0f);
-        k_cache_render<VBT_CACHE_STRIDE576>
-            <<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        k_cache_render<VBT_CACHE_STRIDE576><<<blocks_for(a->controlsPerFrame, CACHE_BLOCK), CACHE_BLOCK, 0, s>>>(
+            ad, a->d_blockLeaf, t0, nt, d_out, a->controlsPerFrame, a->d_err);
     }
     HIPCHK(hipGetLastError());
-    if (a->bad) return check_err_flag(a, s, "frame cache");
+    if (a->bad) return report_bad(a, "frame cache");
     return VBT_OK;
 }
 
@@ -1257,12 +1368,11 @@ int vbt_decode_frame_dense(vbt_asset a, int t, const vbt_compose* c, float* d_ou
     if (a->h.profileType == 2 && (t < 0 || (uint32_t)t >= a->h.frames))
         return vbt_fail(VBT_ERR_INVALID, "scientific frame index out of range");
     hipStream_t s = S(stream);
-    if (a->bad) k_zero_u32<<<1, 64, 0, s>>>(a->d_err, 1);
     const AssetDev ad = a->dev();
     if (a->h.profileType == 2) k_sci_leaf<true><<<dim3(a->A, 1), 512, 0, s>>>(ad, t, 1, d_out, 0, a->d_err);
     else k_dense_render<<<a->h.leafCount, 512, 0, s>>>(ad, t, to_dev(c), d_out, a->d_err);
     HIPCHK(hipGetLastError());
-    if (a->bad) return check_err_flag(a, s, "dense frame");
+    if (a->bad) return report_bad(a, "dense frame");
     return VBT_OK;
 }
 
