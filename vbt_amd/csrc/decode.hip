@@ -418,43 +418,56 @@ __device__ __forceinline__ void sweep_key(const uint8_t* p, uint32_t i, int& f, 
 
 // Sweep of one control over frames t0..t0+nt-1 (out already offset to the
 // slot; consecutive frames are `stride` floats apart).  `keys` points at the
-// control's first keyframe record (LDS or global; the caller instantiates
-// both so that the LDS path compiles to ds_read and the loop never waits for
-// its own outstanding global stores).  Requires strictly increasing frames.
-// The key after next is prefetched at every advance so its LDS latency
-// overlaps the following frames; `rtab[b]` = RN64(1/b) for gaps b < 256.
+// control's first keyframe record in LDS.  Requires strictly increasing key
+// frames.  Invariant after each frame: the next key's frame fn exceeds the
+// frame just produced.  Frames advance by at most one per step, so a single
+// predicated advance per step keeps the invariant (the next key is beyond
+// fn because frames strictly increase); an initial catch-up establishes it
+// for t0.  The loop body is branch-free: the key after next is read every
+// step and selected on advance; rtab[b] = RN64(1/b) for gaps b < 256.
 template <uint32_t KB>
 __device__ __forceinline__ void sweep_slot(const uint8_t* keys, uint32_t keyCount, int F, int t0, int nt, float* out,
                                            uint64_t stride, const double* rtab)
 {
     const uint32_t last = keyCount - 1u;
+    auto recip = [&](int gap) -> double {
+        if (KB == 3u) return rtab[gap & 255];
+        return gap < 256 ? rtab[gap] : 1.0 / (double)gap;
+    };
     uint32_t k = 0;
-    int fk, fn = 0, fp = 0;
-    float vk, vn = 0.0f, vp = 0.0f;
+    int fk, fn = 0;
+    float vk, vn = 0.0f;
     sweep_key<KB>(keys, 0, fk, vk);
     if (last >= 1u) sweep_key<KB>(keys, 1, fn, vn);
-    if (last >= 2u) sweep_key<KB>(keys, 2, fp, vp);
-    auto recip = [&](int gap) { return gap < 256 ? rtab[gap] : 1.0 / (double)gap; };
-    double rinv = last > 0u ? recip(fn - fk) : 0.0;
+    // catch up to the first requested frame
+    const int first = clamp_frame(t0, F);
+    while (k < last && fn <= first) {
+        ++k;
+        fk = fn;
+        vk = vn;
+        if (k < last) sweep_key<KB>(keys, k + 1u, fn, vn);
+    }
+    double rinv = k < last ? recip(fn - fk) : 0.0;
+    int fp;
+    float vp;
+    sweep_key<KB>(keys, k + 2u <= last ? k + 2u : last, fp, vp);
+    const int fmax = F - 1 > 0 ? F - 1 : 0;
     for (int i = 0; i < nt; ++i) {
-        const int frame = clamp_frame(t0 + i, F);
-        while (k < last && fn <= frame) {
-            ++k;
-            fk = fn;
-            vk = vn;
-            fn = fp;
-            vn = vp;
-            if (k < last) rinv = recip(fn - fk);
-            if (k + 2u <= last) sweep_key<KB>(keys, k + 2u, fp, vp);
-        }
-        float v;
-        if (fk >= frame || k == last) {
-            v = vk;
-        } else {
-            const float alpha = sweep_alpha(frame - fk, rinv);
-            v = vk + alpha * (vn - vk);
-        }
-        out[(uint64_t)i * stride] = v;
+        const int tt = t0 + i;
+        const int frame = tt < fmax ? (tt < 0 ? 0 : tt) : fmax;
+        const bool adv = (k < last) & (fn <= frame);
+        const int gapNew = fp - fn;
+        fk = adv ? fn : fk;
+        vk = adv ? vn : vk;
+        fn = adv ? fp : fn;
+        vn = adv ? vp : vn;
+        k += adv ? 1u : 0u;
+        const double rNew = recip(gapNew > 0 ? gapNew : 1);
+        rinv = adv ? rNew : rinv;
+        sweep_key<KB>(keys, k + 2u <= last ? k + 2u : last, fp, vp);
+        const float alpha = sweep_alpha(frame - fk, rinv);
+        const float lerp = vk + alpha * (vn - vk);
+        out[(uint64_t)i * stride] = (fk >= frame || k == last) ? vk : lerp;
     }
 }
 
