@@ -158,6 +158,9 @@ def main():
     info = asset.info
     W, H, D, F = asset.dims
     nt = F
+    best, tile_ms = asset.autotune(nt, reps=2)
+    shapes = hip.Asset.tile_shapes()
+    print(f"[bench] autotune: best tile {shapes[best]} ms per tile {tile_ms}", file=sys.stderr, flush=True)
     out = hip.DeviceArray((nt, info.controlsPerFrame), np.float32, s)
     for _ in range(args.warmup):
         asset.frame_cache_into(out, 0, nt)
@@ -200,7 +203,9 @@ def main():
         "config": {"workload": f"{args.workload}: frame-cache build {W}x{H}x{D} x {F} frames, all frames per step",
                    "active_leaves": info.activeLeaves, "leaves": info.header.leafCount,
                    "controls_per_frame": info.controlsPerFrame, "payload_mb": round(info.payloadBytes / 1e6, 2),
-                   "fast_leaves": info.fast_leaves},
+                   "fast_leaves": info.fast_leaves,
+                   "tile": {"workgroup_slots": shapes[best][0], "lds_window_bytes": shapes[best][1],
+                            "autotune_ms": dict(zip([f"{b}x{w}" for b, w in shapes], tile_ms))}},
         "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

@@ -236,6 +236,19 @@ enum {
  * active leaf is poisoned (its slots are NaN, the others are valid). */
 int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out, void* stream);
 
+/* Autotuning of the frame-cache kernel's tile shape (workgroup slots x LDS
+ * keyframe window).  vbt_asset_autotune times every shape on this asset for
+ * nt frames (reps launches each, into library scratch memory), keeps the
+ * fastest and reports the per-shape times (msPerTile has vbt_cache_tile_count()
+ * entries; may be NULL).  The choice is per asset and can be set directly
+ * from a stored tuning table.  No counterpart in the reference (its cache
+ * shader has one fixed shape, vbt_cache.comp:3). */
+int vbt_cache_tile_count(void);
+int vbt_cache_tile_shape(int tile, uint32_t* block, uint32_t* window);
+int vbt_asset_set_cache_tile(vbt_asset a, int tile);
+int vbt_asset_get_cache_tile(vbt_asset a, int* tile);
+int vbt_asset_autotune(vbt_asset a, int nt, int reps, float* msPerTile, int* best, void* stream);
+
 /* --------------------------------------------------- voxels & queries -- */
 enum {
     VBT_SIGN_NONE = 0,        /* render_temporal_vbt_to_vdb default (optional shellEmptyScale) */

@@ -437,3 +437,54 @@ def test_host_format_functions_agree_with_oracle():
     rng = np.random.default_rng(1)
     for v in rng.standard_normal(5000).astype(np.float32) * 100:
         assert L.vbt_float_to_half_bits(float(v)) == O().vbto_float_to_half_bits(float(v))
+
+
+def simulate_cache_windows(klo, khi, pending, block, window):
+    """Host model of k_cache_render's LDS window passes (same arithmetic):
+    returns the passes as (w0, nw, served lanes)."""
+    klo = np.asarray(klo, np.int64)
+    khi = np.asarray(khi, np.int64)
+    pending = np.asarray(pending, bool).copy()
+    n = len(klo)
+    wave_fits = np.zeros(n, bool)
+    wave_hi = np.zeros(n, np.int64)
+    for w0 in range(0, n, 64):
+        sl = slice(w0, min(n, w0 + 64))
+        p = pending[sl]
+        lo = klo[sl][p].min() if p.any() else 0xFFFFFFFF
+        hi = khi[sl][p].max() if p.any() else 0
+        wave_hi[sl] = hi
+        wave_fits[sl] = lo != 0xFFFFFFFF and hi - (lo & ~3) <= window
+    passes = []
+    for _ in range(n + 2):
+        if not pending.any():
+            return passes
+        lo = int(klo[pending].min())
+        w0 = lo // 4
+        limit = w0 * 4 + window
+        mine = pending & (np.where(wave_fits, wave_hi <= limit, khi <= limit) | (klo == lo))
+        hi = int(khi[mine].max()) if mine.any() else 0
+        nw = (hi + 3) // 4 - w0 if hi > w0 * 4 else 0
+        passes.append((w0, nw, np.nonzero(mine)[0]))
+        pending &= ~mine
+    raise AssertionError("window passes did not terminate")
+
+
+@pytest.mark.parametrize("block,window", [(64, 8192), (64, 16384), (128, 16384), (128, 32768), (256, 32768),
+                                          (256, 49152)])
+def test_cache_window_passes_stay_in_bounds(block, window):
+    rng = np.random.default_rng(block * 7 + window)
+    for trial in range(400):
+        n = block
+        kb = int(rng.choice([3, 8]))
+        counts = rng.integers(0, 256, n) if trial % 3 else rng.integers(0, 60, n)
+        start = int(rng.integers(0, 4096))
+        lo = start + np.concatenate([[0], np.cumsum(counts[:-1] * kb)]) + rng.integers(0, 4, n) * (trial % 2)
+        hi = lo + counts * kb
+        pending = (counts > 0) & (rng.random(n) < 0.95)
+        served = np.zeros(n, int)
+        for w0, nw, lanes in simulate_cache_windows(lo, hi, pending, block, window):
+            assert 0 < nw <= window // 4
+            assert (lo[lanes] >= w0 * 4).all() and (hi[lanes] <= (w0 + nw) * 4).all()
+            served[lanes] += 1
+        np.testing.assert_array_equal(served, pending.astype(int))

@@ -62,6 +62,9 @@ constexpr uint32_t FLAG_SWEEP = 1u << 8;  // every control's keyframe frames str
 constexpr uint32_t STATUS_MASK = 0xFFu;
 constexpr uint32_t NO_LEAF = 0xffffffffu;
 
+inline hipStream_t S(void* s) { return (hipStream_t)s; }
+inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
 struct DevLeaf {
     uint32_t leaf, word, bytes, status;
     uint32_t ncache, cc;       // cache slots; coarse slots
@@ -471,71 +474,89 @@ __device__ __forceinline__ void sweep_slot(const uint8_t* keys, uint32_t keyCoun
     }
 }
 
-// Workgroup size of the cache kernel (the block -> first-leaf table is built
-// for it at asset open) and its LDS keyframe window.
-constexpr uint32_t CACHE_BLOCK = 128;
-constexpr uint32_t CACHE_KEY_LDS = 16384;
-constexpr uint32_t CACHE_WAVES = CACHE_BLOCK / 64;
+// The packed cache is processed in tiles of BLOCK consecutive slots; the
+// first active leaf of every 64-slot group is tabulated at asset open so a
+// workgroup of any size (a multiple of 64) finds its leaf with one load.
+constexpr uint32_t CACHE_GROUP = 64;
 
-// first active leaf of every CACHE_BLOCK-slot tile of the packed cache
-__global__ void k_block_leaf(const uint64_t* ctrl, uint32_t A, uint64_t total, uint32_t nb, uint32_t* blockLeaf)
+__global__ void k_group_leaf(const uint64_t* ctrl, uint32_t A, uint64_t total, uint32_t ng, uint32_t* groupLeaf)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > nb) return;
-    const uint64_t g = (uint64_t)b * CACHE_BLOCK;
-    blockLeaf[b] = g < total ? find_leaf(ctrl, A, g) : (A ? A - 1u : 0u);
+    if (b > ng) return;
+    const uint64_t g = (uint64_t)b * CACHE_GROUP;
+    groupLeaf[b] = g < total ? find_leaf(ctrl, A, g) : (A ? A - 1u : 0u);
 }
 
+template <uint32_t NW>
 __device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t* sm, uint32_t lane, uint32_t wave)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, 64));
+    if (NW == 1) return v;
     if (lane == 0) sm[wave] = v;
     __syncthreads();
     v = sm[0];
 #pragma unroll
-    for (uint32_t w = 1; w < CACHE_WAVES; ++w) v = min(v, sm[w]);
+    for (uint32_t w = 1; w < NW; ++w) v = min(v, sm[w]);
     __syncthreads();
     return v;
 }
 
+template <uint32_t NW>
 __device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* sm, uint32_t lane, uint32_t wave)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    if (NW == 1) return v;
     if (lane == 0) sm[wave] = v;
     __syncthreads();
     v = sm[0];
 #pragma unroll
-    for (uint32_t w = 1; w < CACHE_WAVES; ++w) v = max(v, sm[w]);
+    for (uint32_t w = 1; w < NW; ++w) v = max(v, sm[w]);
     __syncthreads();
     return v;
 }
 
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    return v;
+}
+
 // One thread per (active leaf, cache slot): the thread produces that control
-// for frames t0..t0+nt-1.  The workgroup reads its first leaf from the block
+// for frames t0..t0+nt-1.  The workgroup reads its first leaf from the group
 // table and each thread locates its leaf and stream.  Keyframe records of the
 // workgroup's controls are contiguous in the pool (consecutive controls of a
 // stream store consecutive key ranges; active leaves are in leaf order), so
-// the workgroup stages them into an LDS window with dword loads, in as many
-// passes as the window needs: each pass serves the controls whose records
-// lie inside it.  Leaves whose every control has strictly increasing
-// keyframe frames use the forward sweep (each frame's bracket is then the
-// adjacent key pair around it, exactly the bracket the reference's hinted
-// binary search returns for increasing keys); other leaves run the
-// reference bracket search per frame from global memory.
-template <int LAYOUT>
-__global__ void __launch_bounds__(CACHE_BLOCK) k_cache_render(AssetDev a, const uint32_t* blockLeaf, int t0, int nt,
-                                                              float* out, uint64_t total, uint32_t* errFlag)
+// the workgroup stages them into an LDS window of WINDOW bytes with dword
+// loads, in as many passes as the window needs.  A pass serves whole
+// wavefronts whose records fit (so each wavefront runs its frame loop once);
+// a wavefront whose records exceed the window is served lane by lane.
+// Leaves whose every control has strictly increasing keyframe frames use the
+// forward sweep (each frame's bracket is then the adjacent key pair around
+// it, exactly the bracket the reference's hinted binary search returns for
+// increasing keys); other leaves run the reference bracket search per frame
+// from global memory.  BLOCK and WINDOW are the tile shape the autotuner picks.
+template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
+__global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32_t* groupLeaf, int t0, int nt,
+                                                        float* out, uint64_t total, uint32_t* errFlag)
 {
-    __shared__ uint32_t lds[CACHE_KEY_LDS / 4];
-    __shared__ uint32_t sRed[CACHE_WAVES];
+    constexpr uint32_t NW = BLOCK / 64;
+    __shared__ uint32_t lds[WINDOW / 4];
+    __shared__ uint32_t sRed[NW];
     __shared__ double rtab[256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    for (uint32_t i = tid; i < 256u; i += CACHE_BLOCK) rtab[i] = i ? 1.0 / (double)i : 0.0;
-    const uint64_t g = (uint64_t)blockIdx.x * CACHE_BLOCK + tid;
+    for (uint32_t i = tid; i < 256u; i += BLOCK) rtab[i] = i ? 1.0 / (double)i : 0.0;
+    const uint64_t g = (uint64_t)blockIdx.x * BLOCK + tid;
     const bool live = g < total;
-    uint32_t c = blockLeaf[blockIdx.x];
+    uint32_t c = groupLeaf[(blockIdx.x * BLOCK) / CACHE_GROUP];
     if (live)
         while (c + 1u < a.A && a.ctrl[c + 1u] <= g) ++c;
     const DevLeaf& d = a.leaves[c];
@@ -585,18 +606,23 @@ __global__ void __launch_bounds__(CACHE_BLOCK) k_cache_render(AssetDev a, const 
         }
     }
     bool pending = sweep && writes;
+    // A wavefront "fits" when its records fit in one window starting at the
+    // dword holding its first record; then it is served in a single pass.
+    const uint32_t waveLo = wave_min(pending ? kLo : 0xFFFFFFFFu);
+    const uint32_t waveHi = wave_max(pending ? kHi : 0u);
+    const bool waveFits = waveLo != 0xFFFFFFFFu && waveHi - (waveLo & ~3u) <= WINDOW;
     for (;;) {
-        // window start: the lowest pending record; the window serves every
-        // pending control whose records end inside it
-        const uint32_t lo = block_min(pending ? kLo : 0xFFFFFFFFu, sRed, lane, wave);
+        const uint32_t lo = block_min<NW>(pending ? kLo : 0xFFFFFFFFu, sRed, lane, wave);
         if (lo == 0xFFFFFFFFu) break;
         const uint32_t w0 = lo / 4u;
-        const uint32_t limit = w0 * 4u + CACHE_KEY_LDS;
-        const bool mine = pending && kHi <= limit;
-        const uint32_t hi = block_max(mine ? kHi : 0u, sRed, lane, wave);
-        const uint32_t nw = (hi + 3u) / 4u - w0;
+        const uint32_t limit = w0 * 4u + WINDOW;
+        // progress: the lowest pending record is always served (a single
+        // control's records span at most 255 * 8 bytes < WINDOW)
+        const bool mine = pending && ((waveFits ? waveHi <= limit : kHi <= limit) || kLo == lo);
+        const uint32_t hi = block_max<NW>(mine ? kHi : 0u, sRed, lane, wave);
+        const uint32_t nw = hi > w0 * 4u ? (hi + 3u) / 4u - w0 : 0u;
         const uint32_t* pw = (const uint32_t*)a.payload;
-        for (uint32_t i = tid; i < nw; i += CACHE_BLOCK) lds[i] = pw[w0 + i];
+        for (uint32_t i = tid; i < nw; i += BLOCK) lds[i] = pw[w0 + i];
         __syncthreads();
         if (mine) {
             const uint8_t* p = (const uint8_t*)lds + (kLo - w0 * 4u);
@@ -605,6 +631,35 @@ __global__ void __launch_bounds__(CACHE_BLOCK) k_cache_render(AssetDev a, const 
             pending = false;
         }
         __syncthreads();
+    }
+}
+
+// Tile shapes the autotuner chooses from: (workgroup slots, LDS window bytes).
+struct CacheTile {
+    uint32_t block, window;
+};
+constexpr CacheTile CACHE_TILES[] = {{64, 8192}, {64, 16384}, {128, 16384}, {128, 32768}, {256, 32768}, {256, 49152}};
+constexpr int CACHE_TILE_COUNT = (int)(sizeof(CACHE_TILES) / sizeof(CACHE_TILES[0]));
+constexpr int CACHE_TILE_DEFAULT = 2;
+
+template <int LAYOUT>
+void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* groupLeaf, int t0, int nt, float* out,
+                         uint64_t total, uint32_t* err, hipStream_t s)
+{
+    switch (tile) {
+#define VBT_TILE(i, B, W)                                                                                   \
+    case i:                                                                                                 \
+        k_cache_render<LAYOUT, B, W><<<blocks_for(total, B), B, 0, s>>>(ad, groupLeaf, t0, nt, out, total, err); \
+        break;
+        VBT_TILE(0, 64, 8192)
+        VBT_TILE(1, 64, 16384)
+        VBT_TILE(2, 128, 16384)
+        VBT_TILE(3, 128, 32768)
+        VBT_TILE(4, 256, 32768)
+        VBT_TILE(5, 256, 49152)
+#undef VBT_TILE
+    default:
+        break;
     }
 }
 
@@ -1087,15 +1142,6 @@ __global__ void k_fill(float* p, uint64_t n, float v)
         p[i] = v;
 }
 
-__global__ void k_zero_u32(uint32_t* p, uint32_t n)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
-inline hipStream_t S(void* s) { return (hipStream_t)s; }
-
-inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 } // namespace
 
@@ -1112,7 +1158,8 @@ struct vbt_asset_s {
     uint64_t* d_ctrl = nullptr;
     float* d_basis = nullptr;
     uint32_t* d_err = nullptr;   // error flag word for launches
-    uint32_t* d_blockLeaf = nullptr; // first active leaf of each cache-kernel tile
+    uint32_t* d_blockLeaf = nullptr; // first active leaf of each 64-slot group of the packed cache
+    int cacheTile = CACHE_TILE_DEFAULT;
     Pair* d_blocks = nullptr;
     uint32_t A = 0, bad = 0;
     uint64_t controlsPerFrame = 0, payloadBytes = 0, deviceBytes = 0, fast = 0;
@@ -1272,10 +1319,10 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
     if (a->A) k_validate<<<a->A, 256, 0, s>>>(ad, a->d_leaves, a->d_status);
     TRY(hipGetLastError());
     {
-        const uint32_t nbc = blocks_for(a->controlsPerFrame, CACHE_BLOCK);
-        if ((rc = dalloc(a, &a->d_blockLeaf, (size_t)nbc + 1))) return fail(rc);
-        k_block_leaf<<<blocks_for(nbc + 1ull, 256), 256, 0, s>>>(a->d_ctrl, a->A, a->controlsPerFrame, nbc,
-                                                                  a->d_blockLeaf);
+        const uint32_t ng = blocks_for(a->controlsPerFrame, CACHE_GROUP);
+        if ((rc = dalloc(a, &a->d_blockLeaf, (size_t)ng + 1))) return fail(rc);
+        k_group_leaf<<<blocks_for(ng + 1ull, 256), 256, 0, s>>>(a->d_ctrl, a->A, a->controlsPerFrame, ng,
+                                                                 a->d_blockLeaf);
         TRY(hipGetLastError());
     }
     // host-side statistics of the page table
@@ -1362,13 +1409,13 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     if (a->h.profileType == 2) {
         k_sci_leaf<false><<<dim3(a->A, (uint32_t)nt), 512, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
     } else if (layout == VBT_CACHE_PACKED) {
-        k_cache_render<VBT_CACHE_PACKED><<<blocks_for(a->controlsPerFrame, CACHE_BLOCK), CACHE_BLOCK, 0, s>>>(
-            ad, a->d_blockLeaf, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        launch_cache_render<VBT_CACHE_PACKED>(a->cacheTile, ad, a->d_blockLeaf, t0, nt, d_out, a->controlsPerFrame,
+                                              a->d_err, s);
     } else {
         const uint64_t n = (uint64_t)a->A * 576u * (uint64_t)nt;
         k_fill<<<(uint32_t)std::min<uint64_t>(blocks_for(n, 256), 65535), 256, 0, s>>>(d_out, n, 0.0f);
-        k_cache_render<VBT_CACHE_STRIDE576><<<blocks_for(a->controlsPerFrame, CACHE_BLOCK), CACHE_BLOCK, 0, s>>>(
-            ad, a->d_blockLeaf, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        launch_cache_render<VBT_CACHE_STRIDE576>(a->cacheTile, ad, a->d_blockLeaf, t0, nt, d_out,
+                                                 a->controlsPerFrame, a->d_err, s);
     }
     HIPCHK(hipGetLastError());
     if (a->bad) return report_bad(a, "frame cache");
@@ -1449,6 +1496,73 @@ int vbt_decode_shell_voxel_values(const uint8_t* d_leaf, uint32_t leafBytes, int
     k_shell_values<<<blocks_for(n, 256), 256, 0, S(stream)>>>(d_leaf, leafBytes, totalFrames, d_vox, d_t, n, d_out,
                                                               d_status);
     HIPCHK(hipGetLastError());
+    return VBT_OK;
+}
+
+// -------------------------------------------------------------- autotune --
+int vbt_cache_tile_count(void) { return CACHE_TILE_COUNT; }
+
+int vbt_cache_tile_shape(int tile, uint32_t* block, uint32_t* window)
+{
+    if (tile < 0 || tile >= CACHE_TILE_COUNT) return vbt_fail(VBT_ERR_INVALID, "tile index out of range");
+    *block = CACHE_TILES[tile].block;
+    *window = CACHE_TILES[tile].window;
+    return VBT_OK;
+}
+
+int vbt_asset_set_cache_tile(vbt_asset a, int tile)
+{
+    if (!a || tile < 0 || tile >= CACHE_TILE_COUNT) return vbt_fail(VBT_ERR_INVALID, "tile index out of range");
+    a->cacheTile = tile;
+    return VBT_OK;
+}
+
+int vbt_asset_get_cache_tile(vbt_asset a, int* tile)
+{
+    if (!a) return vbt_fail(VBT_ERR_INVALID, "null asset");
+    *tile = a->cacheTile;
+    return VBT_OK;
+}
+
+int vbt_asset_autotune(vbt_asset a, int nt, int reps, float* msPerTile, int* best, void* stream)
+{
+    if (!a || nt <= 0 || reps <= 0) return vbt_fail(VBT_ERR_INVALID, "invalid autotune request");
+    if (a->h.profileType == 2 || a->A == 0) {
+        if (best) *best = a->cacheTile;
+        return VBT_OK; // scientific caches have a single kernel shape
+    }
+    if ((uint64_t)nt > a->h.frames) nt = (int)a->h.frames;
+    hipStream_t s = S(stream);
+    float* scratch = nullptr;
+    HIPCHK(hipMalloc((void**)&scratch, 4ull * a->controlsPerFrame * (uint64_t)nt));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const AssetDev ad = a->dev();
+    int bestTile = a->cacheTile;
+    float bestMs = 1e30f;
+    for (int tile = 0; tile < CACHE_TILE_COUNT; ++tile) {
+        launch_cache_render<VBT_CACHE_PACKED>(tile, ad, a->d_blockLeaf, 0, nt, scratch, a->controlsPerFrame, a->d_err, s);
+        HIPCHK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r)
+            launch_cache_render<VBT_CACHE_PACKED>(tile, ad, a->d_blockLeaf, 0, nt, scratch, a->controlsPerFrame,
+                                                  a->d_err, s);
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)reps;
+        if (msPerTile) msPerTile[tile] = ms;
+        if (ms < bestMs) {
+            bestMs = ms;
+            bestTile = tile;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    a->cacheTile = bestTile;
+    if (best) *best = bestTile;
     return VBT_OK;
 }
 

@@ -101,6 +101,12 @@ def lib():
         L.vbt_decode_shell_voxel_values.argtypes = [c_void_p, c_u32, c_int, c_int, c_void_p, c_void_p, c_u32,
                                                     c_void_p, c_void_p, c_void_p]
         L.vbt_validate_file.argtypes = [c_void_p, c_u64, ctypes.POINTER(FileHeader)]
+        L.vbt_cache_tile_count.restype = c_int
+        L.vbt_cache_tile_shape.argtypes = [c_int, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]
+        L.vbt_asset_set_cache_tile.argtypes = [c_void_p, c_int]
+        L.vbt_asset_get_cache_tile.argtypes = [c_void_p, ctypes.POINTER(c_int)]
+        L.vbt_asset_autotune.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(c_int), c_void_p]
         L.vbt_half_bits_to_float.restype = ctypes.c_float
         L.vbt_half_bits_to_float.argtypes = [ctypes.c_uint16]
         L.vbt_float_to_half_bits.restype = ctypes.c_uint16
@@ -257,6 +263,36 @@ class Asset:
         d = DeviceArray(self.header.leafCount, np.uint32, self.stream)
         check(lib().vbt_asset_leaf_status(self.h, d.ptr, self.stream.h))
         return d.host()
+
+    # ---- frame-cache kernel tile shape (autotuned per asset) ----
+    @staticmethod
+    def tile_shapes():
+        L = lib()
+        out = []
+        for i in range(L.vbt_cache_tile_count()):
+            b, w = c_u32(), c_u32()
+            check(L.vbt_cache_tile_shape(i, ctypes.byref(b), ctypes.byref(w)))
+            out.append((b.value, w.value))
+        return out
+
+    @property
+    def cache_tile(self) -> int:
+        t = c_int()
+        check(lib().vbt_asset_get_cache_tile(self.h, ctypes.byref(t)))
+        return t.value
+
+    @cache_tile.setter
+    def cache_tile(self, tile: int):
+        check(lib().vbt_asset_set_cache_tile(self.h, int(tile)))
+
+    def autotune(self, nt: int | None = None, reps: int = 3):
+        """Time every tile shape on this asset, keep the fastest; returns
+        (best tile, [ms per tile])."""
+        n = lib().vbt_cache_tile_count()
+        ms = (ctypes.c_float * n)()
+        best = c_int()
+        check(lib().vbt_asset_autotune(self.h, int(nt or self.dims[3]), reps, ms, ctypes.byref(best), self.stream.h))
+        return best.value, [round(float(x), 4) for x in ms]
 
     def frame_cache_into(self, out: DeviceArray, t0: int, nt: int, layout=CACHE_PACKED):
         check(lib().vbt_decode_frame_cache(self.h, t0, nt, layout, out.ptr, self.stream.h))
