@@ -278,6 +278,78 @@ int vbt_query_points(vbt_asset a, const uint32_t* d_queries, uint32_t n, const v
 int vbt_query_points_plain(vbt_asset a, const uint32_t* d_queries, uint32_t n, const vbt_compose* c, float* d_out,
                            int32_t* d_status, void* stream);
 
+/* ----------------------------------------------------------- serving -- */
+/* A continuous-batching decode server over resident assets (no counterpart in
+ * the reference, whose decode functions are called one value at a time from
+ * OpenMP loops).  Requests -- point-query batches and whole-frame packed cache
+ * builds, on any registered asset -- are submitted from any thread (MPSC).
+ * A host dispatcher admits them in arrival order while the device arena, the
+ * HBM budget (resident assets + request outputs) and the in-flight limit
+ * allow, and appends them to a request ring in host-coherent pinned memory.
+ * On the device a persistent loop runs: one scheduler wave polls the ring and
+ * publishes new requests to device memory; every other workgroup claims work
+ * items (a chunk of queries, a tile of cache slots, one scientific leaf) from
+ * the oldest request with unclaimed items, so requests that arrive while
+ * others run start on the CUs as soon as workgroups free up.  The workgroup
+ * finishing a request's last item writes its completion flag to host memory.
+ * The loop exits after idleUs without new requests or after leaseMs, and the
+ * dispatcher relaunches it when work arrives; every device wait is bounded by
+ * the device clock.  Results stay in HBM until fetched or released.
+ * Registering an asset stops the device loop first (the asset table changes
+ * only between launches). */
+typedef struct vbt_server_s* vbt_server;
+
+typedef struct {
+    uint64_t hbmBudgetBytes; /* admission: resident assets + request outputs (default 80 GiB) */
+    uint64_t arenaBytes;     /* device arena for request outputs (default 4 GiB, clipped to the budget) */
+    uint64_t stagingBytes;   /* pinned host staging for query inputs (default 256 MiB) */
+    uint32_t maxInflight;    /* admitted, unfinished requests at once (default 4096) */
+    uint32_t blocksPerCU;    /* persistent workgroups per CU, 256 lanes each (default 2) */
+    uint32_t queryItem;      /* point queries per work item, multiple of 32 (default 128) */
+    uint32_t cacheItem;      /* packed-cache slots per work item, multiple of 256 (default 1024) */
+    uint32_t idleUs;         /* device loop exits after this long without new requests (default 2000) */
+    uint32_t leaseMs;        /* device loop exits (and is relaunched) at least this often (default 250) */
+} vbt_server_config;
+
+typedef struct {
+    uint64_t id;
+    int32_t status;          /* VBT_OK, or VBT_ERR_FORMAT when an element failed (see d_status) */
+    uint32_t count;          /* values */
+    const float* d_values;   /* device pointer, valid until fetch/release */
+    const int32_t* d_status; /* per-query statuses (queries) or NULL */
+    double latencyUs;        /* submit -> completion observed by the dispatcher */
+    uint64_t launch;         /* device-loop launch it ran in */
+} vbt_request_result;
+
+typedef struct {
+    uint64_t submitted, completed, rejected, launches, values, requeued;
+    double p50Us, p99Us, meanUs, maxUs;
+    uint64_t maxQueueDepth, maxInflight, hbmInUse, hbmBudget, arenaInUse;
+} vbt_server_stats;
+
+void vbt_server_default_config(vbt_server_config* c);
+int vbt_server_create(const vbt_server_config* c, vbt_server* out);
+int vbt_server_destroy(vbt_server s);
+/* Register a resident asset (counts against the HBM budget). */
+int vbt_server_register(vbt_server s, vbt_asset a, int* assetId);
+/* Queue a point-query batch (host array of n x {x,y,z,t}; copied to pinned
+ * staging before returning).  VBT_ERR_BUDGET when its buffers can never fit;
+ * VBT_ERR_AGAIN when the staging area stays full for a second. */
+int vbt_server_submit_query(vbt_server s, int assetId, const uint32_t* queries, uint32_t n, const vbt_compose* c,
+                            uint64_t* requestId);
+/* Queue a whole-frame packed cache build of frame t (layout VBT_CACHE_PACKED,
+ * same values as vbt_decode_frame_cache). */
+int vbt_server_submit_cache(vbt_server s, int assetId, int t, uint64_t* requestId);
+/* Wait up to timeoutMs (-1 = forever) for completion; VBT_ERR_AGAIN on timeout. */
+int vbt_server_poll(vbt_server s, uint64_t requestId, int timeoutMs, vbt_request_result* out);
+/* Copy a completed request's values (and per-query statuses, may be NULL) to
+ * host memory and release its device buffers. */
+int vbt_server_fetch(vbt_server s, uint64_t requestId, float* values, int32_t* status);
+int vbt_server_release(vbt_server s, uint64_t requestId);
+/* Latency statistics over completed requests of a type (-1 all, 0 queries, 1 cache builds). */
+int vbt_server_stats_get(vbt_server s, int requestType, vbt_server_stats* out);
+int vbt_server_reset_stats(vbt_server s);
+
 /* ------------------------------------------- device / stream helpers -- */
 int vbt_device_count(int* n);
 int vbt_device_alloc(void** d_ptr, size_t bytes);

@@ -4,6 +4,7 @@
 
   vbt_amd/libvbt_hip.so    HIP decode library (gfx950), C-ABI in include/vbt_hip.h
   vbt_amd/libvbt_synth.so  synthetic fields + asset encoder (host C++, OpenMP)
+  vbt_amd/libvbt_loadgen.so  open-loop load generator over libvbt_hip.so (host C++)
   oracle/liboracle.so      CPU oracle (test infrastructure)
 """
 from __future__ import annotations
@@ -85,6 +86,17 @@ def build_synth():
     return out
 
 
+def build_loadgen():
+    out = os.path.join(HERE, "libvbt_loadgen.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-shared", "-D__HIP_PLATFORM_AMD__",
+          f"-I{ROCM}/include", "-o", tmp, os.path.join(CSRC, "tools", "vbt_loadgen.cpp"),
+          f"-L{HERE}", "-lvbt_hip", f"-L{ROCM}/lib", "-lamdhip64", "-lpthread",
+          "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle")])
     if os.path.isdir("/root/reference"):
@@ -96,9 +108,11 @@ def build_all():
     build_synth()
     build_oracle()
     build_hip()
+    build_loadgen()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "hip": build_hip, "synth": build_synth, "oracle": build_oracle}[w]()
+        {"all": build_all, "hip": build_hip, "synth": build_synth, "oracle": build_oracle,
+         "loadgen": build_loadgen}[w]()

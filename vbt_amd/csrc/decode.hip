@@ -22,6 +22,7 @@
 //   k_control_values / ...           batched forms of the reference per-value decoders
 #include "vbt_format.h"
 #include "vbt_internal.h"
+#include "vbt_kernels.h"
 
 #include <hip/hip_runtime.h>
 
@@ -58,44 +59,9 @@ int vbt_fail(int code, const std::string& msg)
 
 namespace {
 
-constexpr uint32_t FLAG_SWEEP = 1u << 8;  // every control's keyframe frames strictly increase
-constexpr uint32_t STATUS_MASK = 0xFFu;
-constexpr uint32_t NO_LEAF = 0xffffffffu;
 
 inline hipStream_t S(void* s) { return (hipStream_t)s; }
 inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
-
-struct DevLeaf {
-    uint32_t leaf, word, bytes, status;
-    uint32_t ncache, cc;       // cache slots; coarse slots
-    uint32_t lw, lh, ld;       // cropped extent
-    uint32_t sciHeader;        // scientific header word
-    LeafPlan plan;
-};
-
-__host__ __device__ inline void leaf_coords(uint32_t leaf, uint32_t W, uint32_t H, uint32_t D, uint32_t& bx,
-                                            uint32_t& by, uint32_t& bz, uint32_t& lw, uint32_t& lh, uint32_t& ld)
-{
-    const uint32_t lcx = (W + 7u) / 8u, lcy = (H + 7u) / 8u;
-    bx = leaf % lcx;
-    by = (leaf / lcx) % lcy;
-    bz = leaf / (lcx * lcy);
-    lw = W - bx * 8u < 8u ? W - bx * 8u : 8u;
-    lh = H - by * 8u < 8u ? H - by * 8u : 8u;
-    ld = D - bz * 8u < 8u ? D - bz * 8u : 8u;
-}
-
-struct AssetDev {
-    const uint32_t* offsets;
-    const uint8_t* payload;
-    const float* scales;
-    const uint32_t* compact;
-    const DevLeaf* leaves;
-    const uint64_t* ctrl;
-    const float* basis;
-    uint32_t W, H, D, F, L, A, profile, scaleCount;
-    uint64_t payloadBytes;
-};
 
 // ------------------------------------------------------------ wave helpers --
 __device__ inline uint32_t wave_prefix_u32(uint32_t v, uint32_t lane, uint32_t& total)
@@ -361,17 +327,6 @@ __global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, u
     }
 }
 
-__device__ inline uint32_t find_leaf(const uint64_t* ctrl, uint32_t A, uint64_t g)
-{
-    uint32_t lo = 0, hi = A; // ctrl[lo] <= g < ctrl[hi]
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ctrl[mid] <= g) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // ---------------------------------------------------------- cache: render --
 // Lerp weight of the sweep: alpha = (frame - f0) / (f1 - f0) for integer
 // frames 0 <= frame - f0 < f1 - f0 < 2^16.  With r = RN64(1 / (f1 - f0)),
@@ -475,9 +430,9 @@ __device__ __forceinline__ void sweep_slot(const uint8_t* keys, uint32_t keyCoun
 }
 
 // The packed cache is processed in tiles of BLOCK consecutive slots; the
-// first active leaf of every 64-slot group is tabulated at asset open so a
-// workgroup of any size (a multiple of 64) finds its leaf with one load.
-constexpr uint32_t CACHE_GROUP = 64;
+// first active leaf of every CACHE_GROUP-slot group is tabulated at asset
+// open so a workgroup of any size (a multiple of 64) finds its leaf with one
+// load.
 
 __global__ void k_group_leaf(const uint64_t* ctrl, uint32_t A, uint64_t total, uint32_t ng, uint32_t* groupLeaf)
 {
@@ -664,130 +619,6 @@ void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* groupLeaf
 }
 
 // ------------------------------------------------------ scientific decode --
-__device__ inline float sci_control(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t ci, int t)
-{
-    const uint32_t cb = 4u + ci * (h.keep + 1u);
-    const float* basis = a.basis + (size_t)t * 64u;
-    double sum = (double)half_to_float(b.u16(cb)) * (double)basis[0];
-    for (uint32_t k = 1; k < h.keep; ++k) {
-        const float coeff = (float)(int8_t)(uint8_t)b.u8(cb + 2u + (k - 1u)) * a.scales[k - 1u];
-        const double term = (double)coeff * (double)basis[k];
-        sum += term;
-    }
-    return (float)sum;
-}
-
-__device__ inline uint32_t bfp_code(const Bytes& b, uint32_t blockBase, uint32_t ci, uint32_t bits)
-{
-    const uint32_t bit = ci * bits;
-    const uint32_t byteIndex = blockBase + bit / 8u;
-    const uint32_t shift = bit % 8u;
-    uint32_t packed = b.u8(byteIndex);
-    if (shift + bits > 8u) packed |= b.u8(byteIndex + 1u) << 8;
-    return (packed >> shift) & ((1u << bits) - 1u);
-}
-
-__device__ inline float sci_dense_value(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t lx, uint32_t ly,
-                                        uint32_t lz, uint32_t t)
-{
-    const int R = h.mode == 2u ? 3 : 4;
-    const uint32_t bits = h.bits;
-    const uint32_t vals = (uint32_t)(R * R * R);
-    const uint32_t blockBytes = (vals * bits + 7u) / 8u;
-    const uint32_t scalesBase = 4u + 64u * (h.keep + 1u);
-    int x0, y0, z0;
-    float tx, ty, tz;
-    sci_dense_coord(lx, R, x0, tx);
-    sci_dense_coord(ly, R, y0, ty);
-    sci_dense_coord(lz, R, z0, tz);
-    if (h.subtype == 1u) {
-        if (h.tk == 0u || bits == 0u) return 0.0f;
-        const uint32_t dataBase = scalesBase + h.tk * 2u;
-        float residual = 0.0f;
-        for (uint32_t k = 0; k < h.tk; ++k) {
-            const float scale = half_to_float(b.u16(scalesBase + k * 2u));
-            const uint32_t bb = dataBase + k * blockBytes;
-            float s[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
-                s[i] = decode_bfp(bfp_code(b, bb, (uint32_t)((cz * R + cy) * R + cx), bits), scale, (int)bits);
-            }
-            residual += trilinear(s, tx, ty, tz) * a.basis[(size_t)t * 64u + k];
-        }
-        return residual;
-    }
-    const uint32_t dataBase = scalesBase + a.F * 2u;
-    const float scale = half_to_float(b.u16(scalesBase + t * 2u));
-    const uint32_t frameBase = dataBase + t * blockBytes;
-    float s[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
-        s[i] = decode_bfp(bfp_code(b, frameBase, (uint32_t)((cz * R + cy) * R + cx), bits), scale, (int)bits);
-    }
-    return trilinear(s, tx, ty, tz);
-}
-
-__device__ inline float sci_sparse(const Bytes& b, const SciHeader& h, uint32_t idx, uint32_t t)
-{
-    if (h.count == 0u) return 0.0f;
-    const uint32_t cap = h.tier == 0 ? 64u : h.tier == 1 ? 256u : h.tier == 2 ? 768u : 1024u;
-    const uint32_t bin = t / h.fpb < 15u ? t / h.fpb : 15u;
-    const uint32_t localT = (t - bin * h.fpb) & 0xFFu;
-    const uint32_t sparseBase = 4u + 64u * (h.keep + 1u);
-    const float eventScale = bits_to_float(b.u32(sparseBase));
-    if (!(eventScale > 0.0f)) return 0.0f;
-    const uint32_t maskWord = b.u32(sparseBase + 4u + (idx >> 5) * 4u);
-    if ((maskWord & (1u << (idx & 31u))) == 0u) return 0.0f;
-    const uint32_t binsBase = sparseBase + 68u;
-    const uint32_t start = b.u16(binsBase + bin * 2u);
-    const uint32_t end = bin + 1u < 16u ? b.u16(binsBase + (bin + 1u) * 2u) : (h.count & 0xFFFFu);
-    if (start >= end) return 0.0f;
-    const uint32_t coordsBase = binsBase + 32u;
-    const uint32_t residualBase = coordsBase + cap * 2u;
-    const uint32_t target = ((localT << 9) | (idx & 0x1FFu)) & 0xFFFFu;
-    int left = (int)start, right = (int)end - 1;
-    while (left <= right) {
-        const int mid = (left + right) >> 1;
-        const uint32_t key = b.u16(coordsBase + (uint32_t)mid * 2u);
-        if (key == target) {
-            const uint32_t byte = b.u8(residualBase + (uint32_t)(mid >> 1));
-            const uint32_t nib = (mid & 1) == 0 ? (byte & 0xFu) : ((byte >> 4) & 0xFu);
-            int q = (int)nib;
-            if (q & 8) q -= 16;
-            return (float)q * eventScale;
-        }
-        if (key < target) left = mid + 1;
-        else right = mid - 1;
-    }
-    return 0.0f;
-}
-
-__device__ inline float sci_residual(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t lx, uint32_t ly,
-                                     uint32_t lz, uint32_t t)
-{
-    if (h.mode == 0u) return 0.0f;
-    if (h.mode == 1u) return sci_sparse(b, h, (lz * 8u + ly) * 8u + lx, t);
-    return sci_dense_value(a, b, h, lx, ly, lz, t);
-}
-
-__device__ inline float sci_coarse_sample(const float* ctrl, uint32_t lx, uint32_t ly, uint32_t lz)
-{
-    int x0, y0, z0;
-    float tx, ty, tz;
-    sci_coarse_coord(lx, x0, tx);
-    sci_coarse_coord(ly, y0, ty);
-    sci_coarse_coord(lz, z0, tz);
-    float s[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
-        s[i] = ctrl[(cz * 4 + cy) * 4 + cx];
-    }
-    return trilinear(s, tx, ty, tz);
-}
-
 // scientific leaf-frames: one workgroup per (active leaf, frame), 512 lanes = voxels
 template <bool DENSE_OUT>
 __global__ void __launch_bounds__(512) k_sci_leaf(AssetDev a, int t0, int nt, float* out, uint64_t total,
@@ -822,11 +653,6 @@ __global__ void __launch_bounds__(512) k_sci_leaf(AssetDev a, int t0, int nt, fl
 }
 
 // ------------------------------------------------------ render voxels --
-struct ComposeDev {
-    int signRule;
-    float background, shellEmptyScale, shellEmptyMax;
-};
-
 __device__ inline float sample_grid(const float* g, int res, int lw, int lh, int ld, int lx, int ly, int lz)
 {
     if (res < 1) return 0.0f;
@@ -845,26 +671,6 @@ __device__ inline float sample_grid(const float* g, int res, int lw, int lh, int
     s[6] = g[(z1 * res + y1) * res + x0];
     s[7] = g[(z1 * res + y1) * res + x1];
     return trilinear(s, tx, ty, tz);
-}
-
-__device__ inline float apply_sign(const ComposeDev& cm, float v, bool occupied)
-{
-    if (cm.signRule == VBT_SIGN_NONE) {
-        if (!isnan(cm.shellEmptyMax) && cm.shellEmptyScale < 1.0f && !occupied && v > 0.0f && v <= cm.shellEmptyMax)
-            v *= cm.shellEmptyScale;
-    } else if (!occupied && v < 0.0f) {
-        v = cm.signRule == VBT_SIGN_CLAMP_ZERO ? 0.0f : cm.background;
-    }
-    return v;
-}
-
-// shell residual of a voxel (decodeRenderTemporalLeafShellVoxelValue semantics)
-__device__ inline float shell_value(const Bytes& b, const DevLeaf& d, uint32_t vox, int F, int t, bool& occupied)
-{
-    const uint32_t di = shell_desc_index(b, d.plan.occOffset, vox);
-    occupied = di != NO_LEAF;
-    if (!occupied || d.plan.shell.keys == 0u) return 0.0f;
-    return decode_control(b, d.plan.shell, di, F, t);
 }
 
 // whole frame, render route: one workgroup per leaf, 512 lanes = voxels
@@ -912,116 +718,21 @@ __global__ void __launch_bounds__(512) k_dense_render(AssetDev a, int t, Compose
 }
 
 // ----------------------------------------------------------- point queries --
-// render point value (8 corners per grid decoded by the 8 lanes of a group)
-template <int LANES>
-__device__ inline float render_point(const AssetDev& a, const DevLeaf& d, const ComposeDev& cm, uint32_t lx, uint32_t ly,
-                                     uint32_t lz, int t, uint32_t sub)
-{
-    const Bytes b{a.payload + (size_t)d.word * 4u};
-    const int F = (int)a.F;
-    float v = 0.0f;
-    for (int pass = 0; pass < 2; ++pass) {
-        const int res = pass == 0 ? (int)d.plan.coarseRes : (int)d.plan.fineRes;
-        if (pass == 1 && !d.plan.hasFine) break;
-        if (res < 1) continue;
-        const StreamRef& s = pass == 0 ? d.plan.coarse : d.plan.fine;
-        int x0, x1, y0, y1, z0, z1;
-        float tx, ty, tz;
-        locate((float)lx, (int)d.lw, res, x0, x1, tx);
-        locate((float)ly, (int)d.lh, res, y0, y1, ty);
-        locate((float)lz, (int)d.ld, res, z0, z1, tz);
-        float sv[8];
-        if (LANES == 8) {
-            const int cx = (sub & 1u) ? x1 : x0, cy = (sub & 2u) ? y1 : y0, cz = (sub & 4u) ? z1 : z0;
-            const float mine = decode_control(b, s, (uint32_t)((cz * res + cy) * res + cx), F, t);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) sv[i] = __shfl(mine, i, 8);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int cx = (i & 1) ? x1 : x0, cy = (i & 2) ? y1 : y0, cz = (i & 4) ? z1 : z0;
-                sv[i] = decode_control(b, s, (uint32_t)((cz * res + cy) * res + cx), F, t);
-            }
-        }
-        const float g = trilinear(sv, tx, ty, tz);
-        v = pass == 0 ? g : v + g;
-    }
-    if (d.plan.mode == MODE_SHELL) {
-        const uint32_t vox = (lz * 8u + ly) * 8u + lx;
-        bool occ = false;
-        if (!d.plan.hasFine) {
-            v += shell_value(b, d, vox, F, t, occ);
-        } else {
-            occ = shell_desc_index(b, d.plan.occOffset, vox) != NO_LEAF;
-        }
-        v = apply_sign(cm, v, occ);
-    }
-    return v;
-}
-
-template <int LANES>
-__device__ inline float sci_point(const AssetDev& a, const DevLeaf& d, uint32_t lx, uint32_t ly, uint32_t lz, int t,
-                                  uint32_t sub)
-{
-    const Bytes b{a.payload + (size_t)d.word * 4u};
-    const SciHeader h = sci_header(d.sciHeader);
-    int x0, y0, z0;
-    float tx, ty, tz;
-    sci_coarse_coord(lx, x0, tx);
-    sci_coarse_coord(ly, y0, ty);
-    sci_coarse_coord(lz, z0, tz);
-    float sv[8];
-    if (LANES == 8) {
-        const int cx = x0 + (int)(sub & 1u), cy = y0 + (int)((sub >> 1) & 1u), cz = z0 + (int)((sub >> 2) & 1u);
-        const float mine = sci_control(a, b, h, (uint32_t)((cz * 4 + cy) * 4 + cx), t);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) sv[i] = __shfl(mine, i, 8);
-    } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
-            sv[i] = sci_control(a, b, h, (uint32_t)((cz * 4 + cy) * 4 + cx), t);
-        }
-    }
-    return trilinear(sv, tx, ty, tz) + sci_residual(a, b, h, lx, ly, lz, (uint32_t)t);
-}
-
 template <int LANES>
 __global__ void __launch_bounds__(256) k_query(AssetDev a, const uint4* q, uint32_t n, ComposeDev cm, float* out,
-                                               int32_t* status, uint32_t* errFlag)
+                                               int32_t* status)
 {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t qi = gid / LANES;
     const uint32_t sub = (uint32_t)(gid % LANES);
-    // groups past the end still take part in the 8-lane shuffles, with a clamped query
+    // groups past the end still take part in the lane shuffles, with a clamped query
     const bool valid = qi < n;
     const uint4 r = q[valid ? qi : (n - 1u)];
-    int32_t st = 0;
-    float v = 0.0f;
-    if (r.x >= a.W || r.y >= a.H || r.z >= a.D) {
-        st = VBT_QUERY_ERR_OUTSIDE;
-    } else if (a.profile == 2u && r.w >= a.F) {
-        st = VBT_QUERY_ERR_FRAME;
-    } else {
-        const uint32_t lcx = (a.W + 7u) / 8u, lcy = (a.H + 7u) / 8u;
-        const uint32_t leaf = ((r.z / 8u) * lcy + r.y / 8u) * lcx + r.x / 8u;
-        const uint32_t c = a.compact[leaf];
-        const int t = r.w > 0x7fffffffu ? 0x7fffffff : (int)r.w;
-        if (c != NO_LEAF) {
-            const DevLeaf& d = a.leaves[c];
-            if ((d.status & STATUS_MASK) != LEAF_OK) {
-                st = (int32_t)(d.status & STATUS_MASK);
-            } else if (a.profile == 2u) {
-                v = sci_point<LANES>(a, d, r.x & 7u, r.y & 7u, r.z & 7u, t, sub);
-            } else {
-                v = render_point<LANES>(a, d, cm, r.x & 7u, r.y & 7u, r.z & 7u, t, sub);
-            }
-        }
-    }
+    int32_t st;
+    const float v = point_query<LANES>(a, cm, r, sub, st);
     if (valid && sub == 0u) {
         out[qi] = st ? __builtin_nanf("") : v;
         if (status) status[qi] = st;
-        if (st) atomicOr(errFlag, 1u);
     }
 }
 
@@ -1185,6 +896,8 @@ struct vbt_asset_s {
         a.profile = h.profileType;
         a.scaleCount = h.coarseAcScaleCount;
         a.payloadBytes = payloadBytes;
+        a.groupLeaf = d_blockLeaf;
+        a.controlsPerFrame = controlsPerFrame;
         return a;
     }
 };
@@ -1231,6 +944,12 @@ ComposeDev to_dev(const vbt_compose* c)
 }
 
 } // namespace
+
+// accessors for the serving layer (server.hip)
+vbtk::AssetDev vbt_internal_asset_dev(vbt_asset a) { return a->dev(); }
+uint64_t vbt_internal_asset_bytes(vbt_asset a) { return a->deviceBytes; }
+int vbt_internal_asset_device(vbt_asset a) { return a->device; }
+uint32_t vbt_internal_asset_bad(vbt_asset a) { return a->bad; }
 
 extern "C" {
 
@@ -1443,8 +1162,8 @@ static int launch_query(vbt_asset a, const uint32_t* d_q, uint32_t n, const vbt_
     if (n == 0) return VBT_OK;
     hipStream_t s = S(stream);
     const AssetDev ad = a->dev();
-    if (plain) k_query<1><<<blocks_for(n, 256), 256, 0, s>>>(ad, (const uint4*)d_q, n, to_dev(c), d_out, d_status, a->d_err);
-    else k_query<8><<<blocks_for(8ull * n, 256), 256, 0, s>>>(ad, (const uint4*)d_q, n, to_dev(c), d_out, d_status, a->d_err);
+    if (plain) k_query<1><<<blocks_for(n, 256), 256, 0, s>>>(ad, (const uint4*)d_q, n, to_dev(c), d_out, d_status);
+    else k_query<8><<<blocks_for(8ull * n, 256), 256, 0, s>>>(ad, (const uint4*)d_q, n, to_dev(c), d_out, d_status);
     HIPCHK(hipGetLastError());
     return VBT_OK;
 }

@@ -50,6 +50,26 @@ class Compose(ctypes.Structure):
                 ("shellEmptyMax", ctypes.c_float)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbmBudgetBytes", c_u64), ("arenaBytes", c_u64), ("stagingBytes", c_u64),
+                ("maxInflight", c_u32), ("blocksPerCU", c_u32), ("queryItem", c_u32), ("cacheItem", c_u32),
+                ("idleUs", c_u32), ("leaseMs", c_u32)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("id", c_u64), ("status", ctypes.c_int32), ("count", c_u32), ("d_values", c_void_p),
+                ("d_status", c_void_p), ("latencyUs", ctypes.c_double), ("launch", c_u64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, c_u64) for n in ("submitted", "completed", "rejected", "launches", "values", "requeued")] + \
+               [(n, ctypes.c_double) for n in ("p50Us", "p99Us", "meanUs", "maxUs")] + \
+               [(n, c_u64) for n in ("maxQueueDepth", "maxInflight", "hbmInUse", "hbmBudget", "arenaInUse")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 def compose(sign_rule=SIGN_NONE, background=0.0, shell_empty_scale=1.0, shell_empty_max=math.nan) -> Compose:
     return Compose(sign_rule, background, shell_empty_scale, shell_empty_max)
 
@@ -107,6 +127,19 @@ def lib():
         L.vbt_asset_get_cache_tile.argtypes = [c_void_p, ctypes.POINTER(c_int)]
         L.vbt_asset_autotune.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(c_int), c_void_p]
+        L.vbt_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
+        L.vbt_server_default_config.restype = None
+        L.vbt_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(c_void_p)]
+        L.vbt_server_destroy.argtypes = [c_void_p]
+        L.vbt_server_register.argtypes = [c_void_p, c_void_p, ctypes.POINTER(c_int)]
+        L.vbt_server_submit_query.argtypes = [c_void_p, c_int, c_void_p, c_u32, ctypes.POINTER(Compose),
+                                              ctypes.POINTER(c_u64)]
+        L.vbt_server_submit_cache.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_u64)]
+        L.vbt_server_poll.argtypes = [c_void_p, c_u64, c_int, ctypes.POINTER(RequestResult)]
+        L.vbt_server_fetch.argtypes = [c_void_p, c_u64, c_void_p, c_void_p]
+        L.vbt_server_release.argtypes = [c_void_p, c_u64]
+        L.vbt_server_stats_get.argtypes = [c_void_p, c_int, ctypes.POINTER(ServerStats)]
+        L.vbt_server_reset_stats.argtypes = [c_void_p]
         L.vbt_half_bits_to_float.restype = ctypes.c_float
         L.vbt_half_bits_to_float.argtypes = [ctypes.c_uint16]
         L.vbt_float_to_half_bits.restype = ctypes.c_uint16
@@ -370,3 +403,78 @@ def decode_leaf_values(leaf_bytes: bytes, total_frames, idx, t, use_fine=False, 
         check(lib().vbt_decode_leaf_control_values(d_leaf.ptr, len(leaf_bytes), int(use_fine), 1, total_frames,
                                                    d_i.ptr, d_t.ptr, len(idx), out.ptr, st.ptr, s.h))
     return out.host(), st.host()
+
+
+REQ_QUERY, REQ_CACHE = 0, 1
+
+
+class Server:
+    """Continuous-batching decode server (vbt_server_* in the C-ABI)."""
+
+    def __init__(self, **cfg):
+        c = ServerConfig()
+        lib().vbt_server_default_config(ctypes.byref(c))
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise TypeError(f"unknown server option {k}")
+            setattr(c, k, v)
+        self.config = c
+        self.h = c_void_p()
+        check(lib().vbt_server_create(ctypes.byref(c), ctypes.byref(self.h)))
+        self._assets = []
+
+    def register(self, asset: Asset) -> int:
+        i = c_int(-1)
+        check(lib().vbt_server_register(self.h, asset.h, ctypes.byref(i)))
+        self._assets.append(asset)
+        return i.value
+
+    def submit_query(self, asset_id: int, q: np.ndarray, comp: Compose | None = None) -> int:
+        q = np.ascontiguousarray(q, dtype=np.uint32).reshape(-1, 4)
+        rid = c_u64(0)
+        check(lib().vbt_server_submit_query(self.h, asset_id, q.ctypes.data, len(q),
+                                            ctypes.byref(comp) if comp is not None else None, ctypes.byref(rid)))
+        return rid.value
+
+    def submit_cache(self, asset_id: int, t: int) -> int:
+        rid = c_u64(0)
+        check(lib().vbt_server_submit_cache(self.h, asset_id, t, ctypes.byref(rid)))
+        return rid.value
+
+    def poll(self, rid: int, timeout_ms: int = -1) -> RequestResult:
+        r = RequestResult()
+        check(lib().vbt_server_poll(self.h, rid, timeout_ms, ctypes.byref(r)))
+        return r
+
+    def fetch(self, rid: int, with_status=False, timeout_ms: int = 60000):
+        """Values (and per-query statuses) of a completed request; releases it.
+        Element failures do not raise: they show as NaN values and statuses."""
+        r = self.poll(rid, timeout_ms)
+        vals = np.empty(r.count, np.float32)
+        st = np.zeros(r.count, np.int32)
+        rc = lib().vbt_server_fetch(self.h, rid, vals.ctypes.data, st.ctypes.data)
+        if rc not in (VBT_OK, 2):
+            check(rc)
+        return (vals, st, r) if with_status else vals
+
+    def release(self, rid: int):
+        check(lib().vbt_server_release(self.h, rid))
+
+    def stats(self, request_type: int = -1) -> dict:
+        s = ServerStats()
+        check(lib().vbt_server_stats_get(self.h, request_type, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        check(lib().vbt_server_reset_stats(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vbt_server_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
