@@ -7,8 +7,10 @@ empty leaves), VBTPACK4-encoded by this repository's encoder, resident in
 HBM.  One step = decode every frame (t = 0..119) of every active leaf into the
 packed compact active-leaf cache (meaningful controls only).  Also reported:
 point-query throughput/latency of the batched kernel vs the plain
-one-request-per-launch GPU decode, and the CPU oracle's decode rate on the
-host cores (bounded sample).
+one-request-per-launch GPU decode, the serving loop (five more resident assets:
+the three fire fields, the liquid level set and the scientific field; Poisson
+and saturation traces against one-request-per-launch decode), and the CPU
+oracle's decode rate on the host cores (bounded sample).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -128,6 +130,54 @@ def query_bench(asset, hip, n_batch: int, reps: int) -> dict:
     return res
 
 
+def serving_bench(hip, loadgen, assets: dict, requests: int, seed: int = 0x5EED0005) -> dict:
+    """Serving loop over several resident assets: an open-loop trace of
+    point-query batches (log-uniform 1..65536 queries, uniform x,y,z,t) and
+    whole-frame cache builds (5% of requests, random t), replayed (a) into the
+    device-resident server and (b) one request per launch (plain kernel, one
+    thread per query), at saturation and at Poisson arrival rates set from the
+    plain path's saturation rate.  Latency: arrival -> results in HBM."""
+    names = list(assets)
+    handles = [assets[n] for n in names]
+    srv = hip.Server()  # 80 GiB HBM budget (default)
+    ids = [srv.register(a) for a in handles]
+    cf = 0.05
+
+    def short(r):
+        return {"qps": round(r["queriesPerSec"], 1), "req_per_s": round(r["requestsPerSec"], 1),
+                "p50_us": round(r["p50Us"], 1), "p99_us": round(r["p99Us"], 1),
+                "query_p50_us": round(r["qp50Us"], 1), "query_p99_us": round(r["qp99Us"], 1),
+                "cache_p50_us": round(r["cp50Us"], 1), "cache_p99_us": round(r["cp99Us"], 1),
+                "failed": r["failed"]}
+
+    plain_sat = loadgen.run(handles, loadgen.MODE_PLAIN, 0.0, requests, cf, 65536, seed)
+    batched_sat = loadgen.run(handles, loadgen.MODE_PER_LAUNCH_BATCHED, 0.0, requests, cf, 65536, seed)
+    server_sat = loadgen.run(handles, loadgen.MODE_SERVER, 0.0, requests, cf, 65536, seed, srv, ids)
+    server_q = loadgen.run(handles, loadgen.MODE_SERVER, 0.0, requests, 0.0, 65536, seed + 1, srv, ids)
+    plain_q = loadgen.run(handles, loadgen.MODE_PLAIN, 0.0, requests, 0.0, 65536, seed + 1)
+    poisson = []
+    for frac in (0.5, 0.9, 1.5):
+        rate = frac * plain_sat["requestsPerSec"]
+        sv = loadgen.run(handles, loadgen.MODE_SERVER, rate, 2 * requests, cf, 65536, seed + 2, srv, ids)
+        pl = loadgen.run(handles, loadgen.MODE_PLAIN, rate, 2 * requests, cf, 65536, seed + 2)
+        poisson.append({"offered_req_per_s": round(rate, 1), "of_plain_saturation": frac,
+                        "server": short(sv), "plain": short(pl)})
+    st = srv.stats()
+    srv.close()
+    return {
+        "assets": {n: {"dims": list(a.dims), "active_leaves": a.info.activeLeaves,
+                       "device_mb": round(a.info.deviceBytes / 1e6, 1)} for n, a in zip(names, handles)},
+        "trace": {"requests": requests, "cache_fraction": cf, "query_batch": "log-uniform 1..65536",
+                  "hbm_budget_gb": round(st["hbmBudget"] / 2**30, 1), "hbm_in_use_gb": round(st["hbmInUse"] / 2**30, 3)},
+        "saturation": {"server": short(server_sat), "plain": short(plain_sat), "per_launch_batched": short(batched_sat),
+                       "server_over_plain_qps": round(server_sat["queriesPerSec"] / plain_sat["queriesPerSec"], 2)},
+        "saturation_queries_only": {"server": short(server_q), "plain": short(plain_q),
+                                    "server_over_plain_qps": round(server_q["queriesPerSec"] / plain_q["queriesPerSec"], 2)},
+        "poisson": poisson,
+        "server_launches": st["launches"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,13 +186,17 @@ def main():
     ap.add_argument("--workload", default="smoke256")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-assets", default="fire_density,fire_flames,fire_temperature,liquid320,sci192",
+                    help="assets resident beside the frame-cache workload in the serving run")
+    ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the decode shards by leaf or "
               "frame ranges with replicated assets; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from vbt_amd import hip, synth
+    from vbt_amd import hip, loadgen, synth
 
     if hip.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -212,6 +266,19 @@ def main():
                      "bytes_per_launch": byt["total"], "kernel": "k_cache_render<0>"},
         "queries": {"random_65536": qres, "random_256": small},
     }
+    if not args.no_serve:
+        serve = {args.workload: asset}
+        for n in [x for x in args.serve_assets.split(",") if x]:
+            t0 = time.time()
+            serve[n] = hip.Asset(synth.WORKLOADS[n].encode(threads=min(16, os.cpu_count() or 1)))
+            print(f"[bench] serving asset {n}: encoded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+        line["serving"] = serving_bench(hip, loadgen, serve, args.serve_requests)
+        sat = line["serving"]["saturation"]
+        print(f"[bench] serving: server {sat['server']['qps'] / 1e6:.1f} Mq/s vs plain {sat['plain']['qps'] / 1e6:.1f} "
+              f"Mq/s at saturation", file=sys.stderr, flush=True)
+        for n, a in serve.items():
+            if a is not asset:
+                a.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data, F, args.cpu_seconds, min(16, os.cpu_count() or 1))
     asset.close()

@@ -10,12 +10,16 @@
 //
 // Device side (k_serve; one launch serves at most RING requests):
 //   block 0, wave 0   scheduler: polls the host ring tail, copies new ring
-//                     entries into a device-memory mirror, publishes them.
+//                     entries into a device-memory mirror and appends them to
+//                     their class list (point queries / frame builds),
+//                     publishes them.
 //   other blocks      workers: claim items of the oldest request with
-//                     unclaimed items (a per-request atomic counter; the
-//                     global cursor only moves forward), decode the item,
-//                     and count it done; the workgroup that completes a
-//                     request's last item flushes and writes its host flag.
+//                     unclaimed items in a class (a per-request atomic
+//                     counter; the class cursor only moves forward), query
+//                     batches first in 7 of 8 workgroups and frame builds
+//                     first in the 8th, decode the item, and count it done;
+//                     the workgroup that completes a request's last item
+//                     flushes and writes its host flag.
 // Ring entries are never reused within a launch, so every mirror entry and
 // counter is written once before it is published and needs no generation tag.
 // Every wait is bounded: the scheduler leaves on the host stop flag, on ring
@@ -102,11 +106,14 @@ struct HostShared {
 };
 
 // Device-memory loop state, cleared before every launch.
+// Requests are claimed in two classes, each in arrival order: point-query
+// batches (latency first) and frame builds.  The scheduler appends each new
+// request's mirror index to its class list.
 struct DevState {
-    uint32_t published;  // requests visible to workers
-    uint32_t cur;        // lowest request that may still have unclaimed items
+    uint32_t pub[2];     // per class: list entries visible to workers
+    uint32_t cur[2];     // per class: lowest list position that may still have unclaimed items
     uint32_t exit;       // the scheduler has left
-    uint32_t pad;
+    uint32_t pad[3];
 };
 
 struct ServeArgs {
@@ -115,6 +122,7 @@ struct ServeArgs {
     uint32_t* doneHost;  // per ring entry: 1 | err << 1 once complete
     DevState* st;
     DevReq* mirror;
+    uint32_t* lists;     // 2 x RING mirror indices, one list per class
     const AssetDev* assets;
     uint64_t idleTicks, leaseTicks, workerTicks;
     uint32_t queryItem, cacheItem;
@@ -138,31 +146,51 @@ __device__ __forceinline__ T ld_dev(const T* p)
 __device__ __forceinline__ void release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 __device__ __forceinline__ void release_system() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, ""); }
 
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Scheduler: wave 0 of block 0.
 __device__ void serve_scheduler(const ServeArgs& A)
 {
     const uint32_t lane = threadIdx.x;
     const uint64_t start = wall_clock64();
     uint64_t last = start;
-    uint32_t head = 0;
+    uint32_t head = 0, n[2] = {0u, 0u};
+    const uint64_t below = (1ull << lane) - 1ull;
     for (;;) {
         const uint32_t tail0 = ld_sys(&A.hs->tail);
         const uint32_t stop = ld_sys(&A.hs->stop);
         const uint32_t tail = tail0 < RING ? tail0 : RING;
         if (tail > head) {
-            for (uint32_t j = head + lane; j < tail; j += 64u) {
-                const uint32_t* src = (const uint32_t*)(A.ring + j);
-                uint32_t* dst = (uint32_t*)(A.mirror + j);
+            for (uint32_t j0 = head; j0 < tail; j0 += 64u) {
+                const uint32_t j = j0 + lane;
+                const bool act = j < tail;
+                uint32_t type = 0u;
+                if (act) {
+                    const uint32_t* src = (const uint32_t*)(A.ring + j);
+                    uint32_t* dst = (uint32_t*)(A.mirror + j);
 #pragma unroll
-                for (int w = 0; w < 16; ++w) dst[w] = ld_sys(src + w);
-                dst[16] = 0u;
-                dst[17] = 0u;
-                dst[18] = 0u;
-                dst[19] = 0u;
+                    for (int w = 0; w < 16; ++w) dst[w] = ld_sys(src + w);
+                    dst[16] = 0u;
+                    dst[17] = 0u;
+                    dst[18] = 0u;
+                    dst[19] = 0u;
+                    type = dst[0];
+                }
+                // append to the class lists in arrival order (wave ballot + popcount)
+                const uint64_t mq = __ballot(act && type == REQ_QUERY);
+                const uint64_t mc = __ballot(act && type != REQ_QUERY);
+                if (act) {
+                    const uint32_t cls = type == REQ_QUERY ? 0u : 1u;
+                    const uint32_t pos = n[cls] + (uint32_t)__popcll((cls ? mc : mq) & below);
+                    A.lists[cls * RING + pos] = j;
+                }
+                n[0] += (uint32_t)__popcll(mq);
+                n[1] += (uint32_t)__popcll(mc);
             }
-            release_agent();  // mirror entries reach device-coherent memory before the publish
+            release_agent();  // mirror entries and lists reach device-coherent memory before the publish
             if (lane == 0) {
-                __hip_atomic_store(&A.st->published, tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&A.st->pub[0], n[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&A.st->pub[1], n[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(&A.hs->head, tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             }
             head = tail;
@@ -179,9 +207,13 @@ __device__ void serve_scheduler(const ServeArgs& A)
     }
 }
 
-// 4 waves per SIMD: the register budget (128) that holds every item path
-// without spills; blocksPerCU up to 4 keeps the grid co-resident.
-__global__ void __launch_bounds__(SERVE_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4))) k_serve(ServeArgs A)
+// Waves per SIMD the register allocation is sized for (4: 128 VGPRs, every
+// item path without spills).
+#ifndef VBT_SERVE_WAVES
+#define VBT_SERVE_WAVES 4
+#endif
+__global__ void __launch_bounds__(SERVE_BLOCK) __attribute__((amdgpu_waves_per_eu(VBT_SERVE_WAVES, VBT_SERVE_WAVES)))
+k_serve(ServeArgs A)
 {
     if (blockIdx.x == 0) {
         if (threadIdx.x < 64u) serve_scheduler(A);
@@ -197,34 +229,45 @@ __global__ void __launch_bounds__(SERVE_BLOCK) __attribute__((amdgpu_waves_per_e
     for (;;) {
         if (tid == 0) {
             uint32_t cmd = 2u;  // 0 work, 1 leave, 2 poll again
-            const uint32_t r = ld_dev(&A.st->cur);
-            const uint32_t pub = ld_dev(&A.st->published);
-            if (r < pub) {
-                DevReq* R = A.mirror + r;
-                const uint32_t cnt = ld_dev(&R->e.itemCount);
-                const uint32_t k = __hip_atomic_fetch_add(&R->claim, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (k < cnt) {
-                    if (haveR != r) {
-                        const uint32_t* src = (const uint32_t*)&R->e;
-                        uint32_t* dst = (uint32_t*)&sE;
+            // one workgroup in eight looks at frame builds first, the rest at
+            // point queries first; either falls through to the other class
+            const uint32_t firstCls = (blockIdx.x & 7u) == 0u ? 1u : 0u;
+            for (uint32_t i = 0; i < 2u && cmd == 2u; ++i) {
+                const uint32_t cls = firstCls ^ i;
+                uint32_t pos = ld_dev(&A.st->cur[cls]);
+                const uint32_t p = ld_dev(&A.st->pub[cls]);
+                while (pos < p) {  // pos only grows, up to p
+                    const uint32_t r = ld_dev(A.lists + cls * RING + pos);
+                    DevReq* R = A.mirror + r;
+                    const uint32_t cnt = ld_dev(&R->e.itemCount);
+                    const uint32_t k =
+                        __hip_atomic_fetch_add(&R->claim, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (k < cnt) {
+                        if (haveR != r) {
+                            const uint32_t* src = (const uint32_t*)&R->e;
+                            uint32_t* dst = (uint32_t*)&sE;
 #pragma unroll
-                        for (int w = 0; w < 16; ++w) dst[w] = ld_dev(src + w);
-                        haveR = r;
+                            for (int w = 0; w < 16; ++w) dst[w] = ld_dev(src + w);
+                            haveR = r;
+                        }
+                        sR = r;
+                        sK = k;
+                        sErr = 0u;
+                        cmd = 0u;
+                        waitStart = 0;
+                        break;
                     }
-                    sR = r;
-                    sK = k;
-                    sErr = 0u;
-                    cmd = 0u;
-                    waitStart = 0;
-                } else {
-                    __hip_atomic_fetch_max(&A.st->cur, r + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_max(&A.st->cur[cls], pos + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ++pos;
                 }
-            } else {
-                // published is read only after exit is seen set (the branch
-                // waits for the first load), so it holds the final count
+            }
+            if (cmd == 2u) {
+                // the counts are read only after exit is seen set (the branch
+                // waits for that load), so they are final
                 const uint32_t ex = ld_dev(&A.st->exit);
                 const uint64_t t = wall_clock64();
-                if (ex && r >= ld_dev(&A.st->published)) {
+                if (ex && ld_dev(&A.st->cur[0]) >= ld_dev(&A.st->pub[0]) &&
+                    ld_dev(&A.st->cur[1]) >= ld_dev(&A.st->pub[1])) {
                     cmd = 1u;
                 } else if (waitStart == 0) {
                     waitStart = t;
@@ -242,8 +285,16 @@ __global__ void __launch_bounds__(SERVE_BLOCK) __attribute__((amdgpu_waves_per_e
             __syncthreads();
             continue;
         }
-        const uint32_t r = sR, k = sK;
-        const RingEntry& E = sE;
+        // the item's fields are wave-uniform: keep them (and so the asset
+        // descriptor's address) in scalar registers
+        const uint32_t r = uni(sR), k = uni(sK);
+        RingEntry E;
+        {
+            const uint32_t* src = (const uint32_t*)&sE;
+            uint32_t* dst = (uint32_t*)&E;
+#pragma unroll
+            for (int w = 0; w < 16; ++w) dst[w] = uni(src[w]);
+        }
         const AssetDev& a = A.assets[E.asset];
         bool err = false;
         if (E.type == REQ_QUERY) {
@@ -331,7 +382,7 @@ __global__ void __launch_bounds__(SERVE_BLOCK) __attribute__((amdgpu_waves_per_e
 
 __global__ void k_clear_state(DevState* st)
 {
-    if (threadIdx.x == 0) *st = DevState{0u, 0u, 0u, 0u};
+    if (threadIdx.x == 0) *st = DevState{{0u, 0u}, {0u, 0u}, 0u, {0u, 0u, 0u}};
 }
 
 using Clock = std::chrono::steady_clock;
@@ -447,6 +498,7 @@ struct vbt_server_s {
     uint32_t* dDoneHost = nullptr;
     DevState* dState = nullptr;
     DevReq* dMirror = nullptr;
+    uint32_t* dLists = nullptr;
 
     // dispatcher state (guarded by mu)
     std::mutex mu;
@@ -503,6 +555,7 @@ bool vbt_server_s::launch_loop()
     A.doneHost = dDoneHost;
     A.st = dState;
     A.mirror = dMirror;
+    A.lists = dLists;
     A.assets = dAssets;
     A.idleTicks = (uint64_t)(cfg.idleUs * ticksPerUs);
     A.leaseTicks = (uint64_t)(cfg.leaseMs * 1000.0 * ticksPerUs);
@@ -708,6 +761,7 @@ int vbt_server_create(const vbt_server_config* c, vbt_server* out)
     s->arena.init(cfg.arenaBytes);
     SCHK(hipMalloc((void**)&s->dState, sizeof(DevState)));
     SCHK(hipMalloc((void**)&s->dMirror, sizeof(DevReq) * RING));
+    SCHK(hipMalloc((void**)&s->dLists, sizeof(uint32_t) * 2u * RING));
     const unsigned hostFlags = hipHostMallocMapped | hipHostMallocCoherent;
     SCHK(hipHostMalloc((void**)&s->hStage, cfg.stagingBytes, hostFlags));
     SCHK(hipHostGetDevicePointer((void**)&s->dStage, s->hStage, 0));
@@ -755,6 +809,7 @@ int vbt_server_destroy(vbt_server s)
     if (s->dAssets) (void)hipFree(s->dAssets);
     if (s->dState) (void)hipFree(s->dState);
     if (s->dMirror) (void)hipFree(s->dMirror);
+    if (s->dLists) (void)hipFree(s->dLists);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->fetchStream) (void)hipStreamDestroy(s->fetchStream);
     delete s;
