@@ -328,107 +328,6 @@ __global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, u
 }
 
 // ---------------------------------------------------------- cache: render --
-// Lerp weight of the sweep: alpha = (frame - f0) / (f1 - f0) for integer
-// frames 0 <= frame - f0 < f1 - f0 < 2^16.  With r = RN64(1 / (f1 - f0)),
-// RN32(RN64((frame - f0) * r)) equals the correctly rounded float quotient the
-// reference computes: the double product is within 2^-52 (relative) of the
-// true quotient, while a quotient a/b with b < 2^16 lies at least 2^-41
-// (relative) away from every float rounding boundary, so both round alike.
-__device__ inline float sweep_alpha(int num, double rinv) { return (float)((double)num * rinv); }
-
-// One thread per (active leaf, cache slot); the thread produces that control
-// for frames t0..t0+nt-1.  The workgroup first locates its leaf range in the
-// page table (two binary searches per workgroup) and stages the leaves'
-// payload bytes -- contiguous in the pool because active leaves are in leaf
-// order -- into LDS with dword loads, so keyframe reads hit LDS.  Leaves
-// whose every control has strictly increasing keyframe frames use a forward
-// sweep over the keyframes: each frame's bracket is then the adjacent key pair
-// around it, which is exactly the bracket the reference's hinted binary
-// search returns for increasing keys.  Other leaves run the reference
-// bracket search per frame.
-
-// Branch-free form of half_to_float for the sweep: the hardware conversion is
-// exact for zeros, subnormals, normals and infinities; NaNs take the
-// reference's bit mapping (payload shifted, no quieting).
-__device__ __forceinline__ float sweep_half(uint32_t h)
-{
-    _Float16 x;
-    const uint16_t hb = (uint16_t)h;
-    __builtin_memcpy(&x, &hb, 2);
-    const float f = (float)x;
-    const uint32_t nanBits = ((h & 0x8000u) << 16) | 0x7F800000u | ((h & 0x3FFu) << 13);
-    return ((h & 0x7C00u) == 0x7C00u && (h & 0x3FFu)) ? __uint_as_float(nanBits) : f;
-}
-
-template <uint32_t KB>
-__device__ __forceinline__ void sweep_key(const uint8_t* p, uint32_t i, int& f, float& v)
-{
-    if (KB == 8u) {
-        const uint8_t* q = p + i * 8u;
-        f = (int)((uint32_t)q[0] | ((uint32_t)q[1] << 8));
-        v = __uint_as_float((uint32_t)q[4] | ((uint32_t)q[5] << 8) | ((uint32_t)q[6] << 16) | ((uint32_t)q[7] << 24));
-    } else {
-        const uint8_t* q = p + i * 3u;
-        f = (int)q[0];
-        v = sweep_half((uint32_t)q[1] | ((uint32_t)q[2] << 8));
-    }
-}
-
-// Sweep of one control over frames t0..t0+nt-1 (out already offset to the
-// slot; consecutive frames are `stride` floats apart).  `keys` points at the
-// control's first keyframe record in LDS.  Requires strictly increasing key
-// frames.  Invariant after each frame: the next key's frame fn exceeds the
-// frame just produced.  Frames advance by at most one per step, so a single
-// predicated advance per step keeps the invariant (the next key is beyond
-// fn because frames strictly increase); an initial catch-up establishes it
-// for t0.  The loop body is branch-free: the key after next is read every
-// step and selected on advance; rtab[b] = RN64(1/b) for gaps b < 256.
-template <uint32_t KB>
-__device__ __forceinline__ void sweep_slot(const uint8_t* keys, uint32_t keyCount, int F, int t0, int nt, float* out,
-                                           uint64_t stride, const double* rtab)
-{
-    const uint32_t last = keyCount - 1u;
-    auto recip = [&](int gap) -> double {
-        if (KB == 3u) return rtab[gap & 255];
-        return gap < 256 ? rtab[gap] : 1.0 / (double)gap;
-    };
-    uint32_t k = 0;
-    int fk, fn = 0;
-    float vk, vn = 0.0f;
-    sweep_key<KB>(keys, 0, fk, vk);
-    if (last >= 1u) sweep_key<KB>(keys, 1, fn, vn);
-    // catch up to the first requested frame
-    const int first = clamp_frame(t0, F);
-    while (k < last && fn <= first) {
-        ++k;
-        fk = fn;
-        vk = vn;
-        if (k < last) sweep_key<KB>(keys, k + 1u, fn, vn);
-    }
-    double rinv = k < last ? recip(fn - fk) : 0.0;
-    int fp;
-    float vp;
-    sweep_key<KB>(keys, k + 2u <= last ? k + 2u : last, fp, vp);
-    const int fmax = F - 1 > 0 ? F - 1 : 0;
-    for (int i = 0; i < nt; ++i) {
-        const int tt = t0 + i;
-        const int frame = tt < fmax ? (tt < 0 ? 0 : tt) : fmax;
-        const bool adv = (k < last) & (fn <= frame);
-        const int gapNew = fp - fn;
-        fk = adv ? fn : fk;
-        vk = adv ? vn : vk;
-        fn = adv ? fp : fn;
-        vn = adv ? vp : vn;
-        k += adv ? 1u : 0u;
-        const double rNew = recip(gapNew > 0 ? gapNew : 1);
-        rinv = adv ? rNew : rinv;
-        sweep_key<KB>(keys, k + 2u <= last ? k + 2u : last, fp, vp);
-        const float alpha = sweep_alpha(frame - fk, rinv);
-        const float lerp = vk + alpha * (vn - vk);
-        out[(uint64_t)i * stride] = (fk >= frame || k == last) ? vk : lerp;
-    }
-}
-
 // The packed cache is processed in tiles of BLOCK consecutive slots; the
 // first active leaf of every CACHE_GROUP-slot group is tabulated at asset
 // open so a workgroup of any size (a multiple of 64) finds its leaf with one
@@ -442,151 +341,53 @@ __global__ void k_group_leaf(const uint64_t* ctrl, uint32_t A, uint64_t total, u
     groupLeaf[b] = g < total ? find_leaf(ctrl, A, g) : (A ? A - 1u : 0u);
 }
 
-template <uint32_t NW>
-__device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t* sm, uint32_t lane, uint32_t wave)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, 64));
-    if (NW == 1) return v;
-    if (lane == 0) sm[wave] = v;
-    __syncthreads();
-    v = sm[0];
-#pragma unroll
-    for (uint32_t w = 1; w < NW; ++w) v = min(v, sm[w]);
-    __syncthreads();
-    return v;
-}
-
-template <uint32_t NW>
-__device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* sm, uint32_t lane, uint32_t wave)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
-    if (NW == 1) return v;
-    if (lane == 0) sm[wave] = v;
-    __syncthreads();
-    v = sm[0];
-#pragma unroll
-    for (uint32_t w = 1; w < NW; ++w) v = max(v, sm[w]);
-    __syncthreads();
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_min(uint32_t v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, 64));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_max(uint32_t v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
-    return v;
-}
-
-// One thread per (active leaf, cache slot): the thread produces that control
-// for frames t0..t0+nt-1.  The workgroup reads its first leaf from the group
-// table and each thread locates its leaf and stream.  Keyframe records of the
-// workgroup's controls are contiguous in the pool (consecutive controls of a
-// stream store consecutive key ranges; active leaves are in leaf order), so
-// the workgroup stages them into an LDS window of WINDOW bytes with dword
-// loads, in as many passes as the window needs.  A pass serves whole
-// wavefronts whose records fit (so each wavefront runs its frame loop once);
-// a wavefront whose records exceed the window is served lane by lane.
-// Leaves whose every control has strictly increasing keyframe frames use the
-// forward sweep (each frame's bracket is then the adjacent key pair around
-// it, exactly the bracket the reference's hinted binary search returns for
-// increasing keys); other leaves run the reference bracket search per frame
-// from global memory.  BLOCK and WINDOW are the tile shape the autotuner picks.
 template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
 __global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32_t* groupLeaf, int t0, int nt,
                                                         float* out, uint64_t total, uint32_t* errFlag)
 {
-    constexpr uint32_t NW = BLOCK / 64;
     __shared__ uint32_t lds[WINDOW / 4];
-    __shared__ uint32_t sRed[NW];
+    __shared__ uint32_t sRed[BLOCK / 64];
     __shared__ double rtab[256];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    for (uint32_t i = tid; i < 256u; i += BLOCK) rtab[i] = i ? 1.0 / (double)i : 0.0;
-    const uint64_t g = (uint64_t)blockIdx.x * BLOCK + tid;
-    const bool live = g < total;
-    uint32_t c = groupLeaf[(blockIdx.x * BLOCK) / CACHE_GROUP];
-    if (live)
-        while (c + 1u < a.A && a.ctrl[c + 1u] <= g) ++c;
-    const DevLeaf& d = a.leaves[c];
-    const uint32_t slot = live ? (uint32_t)(g - a.ctrl[c]) : 0u;
-    const uint32_t status = d.status;
-    const bool isCoarse = slot < d.cc;
-    const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
-    const uint32_t ci = isCoarse ? slot : slot - d.cc;
-    const uint32_t leafByte = d.word * 4u;
-    uint32_t keyStart = 0, keyCount = 0;
-    const bool ok = live && (status & STATUS_MASK) == LEAF_OK;
-    if (ok) {
-        const uint32_t desc = *(const uint32_t*)(a.payload + leafByte + s.base + ci * 4u); // descriptors are 4-aligned
-        keyStart = desc & 0xFFFFu;
-        keyCount = (desc >> 16) & 0xFFu;
-    }
-    const uint32_t kb = keyframe_bytes(s.codec);
-    const bool sweep = ok && keyCount > 0u && (status & FLAG_SWEEP);
-    const uint32_t kLo = leafByte + s.keyBase + keyStart * kb;
-    const uint32_t kHi = kLo + keyCount * kb;
+    for (uint32_t i = threadIdx.x; i < 256u; i += BLOCK) rtab[i] = i ? 1.0 / (double)i : 0.0;
+    cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed, rtab);
+}
 
-    uint64_t stride = 0, dst = 0;
-    bool writes = live;
-    if (LAYOUT == VBT_CACHE_PACKED) {
-        stride = total;
-        dst = g;
-    } else {
+// Few frames: one thread per cache slot, the reference bracket search per
+// frame straight from the payload (about three keyframe records per frame
+// instead of the slot's whole keyframe stream).
+template <int LAYOUT>
+__global__ void __launch_bounds__(256) k_cache_exact(AssetDev a, int t0, int nt, float* out, uint64_t total,
+                                                     uint32_t* errFlag)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (g >= total) return;
+    uint32_t c = a.groupLeaf[g / CACHE_GROUP];
+    while (c + 1u < a.A && a.ctrl[c + 1u] <= g) ++c;
+    const DevLeaf& d = a.leaves[c];
+    const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
+    const bool isCoarse = slot < d.cc;
+    uint64_t stride = total, dst = g;
+    if (LAYOUT != VBT_CACHE_PACKED) {
         const uint32_t s576 = isCoarse ? slot : 64u + (slot - d.cc);
-        if ((isCoarse && slot >= 64u) || s576 >= 576u) writes = false;
+        if ((isCoarse && slot >= 64u) || s576 >= 576u) return;
         stride = (uint64_t)a.A * 576u;
         dst = (uint64_t)c * 576u + s576;
     }
     float* o = out + dst;
-    const int F = (int)a.F;
-    if (writes && !sweep) {
-        if (!ok) {
-            for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = __builtin_nanf("");
-            atomicOr(errFlag, 1u);
-        } else if (keyCount == 0u) {
-            for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = 0.0f;
-        } else {
-            const Bytes b{a.payload + leafByte};
-            const uint32_t binBase = s.base + s.count * 4u + ci * s.bins;
-            for (int i = 0; i < nt; ++i)
-                o[(uint64_t)i * stride] =
-                    decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins, F, t0 + i);
-        }
+    if ((d.status & STATUS_MASK) != LEAF_OK) {
+        for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = __builtin_nanf("");
+        atomicOr(errFlag, 1u);
+        return;
     }
-    bool pending = sweep && writes;
-    // A wavefront "fits" when its records fit in one window starting at the
-    // dword holding its first record; then it is served in a single pass.
-    const uint32_t waveLo = wave_min(pending ? kLo : 0xFFFFFFFFu);
-    const uint32_t waveHi = wave_max(pending ? kHi : 0u);
-    const bool waveFits = waveLo != 0xFFFFFFFFu && waveHi - (waveLo & ~3u) <= WINDOW;
-    for (;;) {
-        const uint32_t lo = block_min<NW>(pending ? kLo : 0xFFFFFFFFu, sRed, lane, wave);
-        if (lo == 0xFFFFFFFFu) break;
-        const uint32_t w0 = lo / 4u;
-        const uint32_t limit = w0 * 4u + WINDOW;
-        // progress: the lowest pending record is always served (a single
-        // control's records span at most 255 * 8 bytes < WINDOW)
-        const bool mine = pending && ((waveFits ? waveHi <= limit : kHi <= limit) || kLo == lo);
-        const uint32_t hi = block_max<NW>(mine ? kHi : 0u, sRed, lane, wave);
-        const uint32_t nw = hi > w0 * 4u ? (hi + 3u) / 4u - w0 : 0u;
-        const uint32_t* pw = (const uint32_t*)a.payload;
-        for (uint32_t i = tid; i < nw; i += BLOCK) lds[i] = pw[w0 + i];
-        __syncthreads();
-        if (mine) {
-            const uint8_t* p = (const uint8_t*)lds + (kLo - w0 * 4u);
-            if (kb == 8u) sweep_slot<8u>(p, keyCount, F, t0, nt, o, stride, rtab);
-            else sweep_slot<3u>(p, keyCount, F, t0, nt, o, stride, rtab);
-            pending = false;
-        }
-        __syncthreads();
-    }
+    const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
+    const uint32_t ci = isCoarse ? slot : slot - d.cc;
+    const Bytes b{a.payload + (size_t)d.word * 4u};
+    const uint32_t desc = b.u32(s.base + ci * 4u);
+    const uint32_t keyStart = desc & 0xFFFFu, keyCount = (desc >> 16) & 0xFFu;
+    const uint32_t binBase = s.base + s.count * 4u + ci * s.bins;
+    for (int i = 0; i < nt; ++i)
+        o[(uint64_t)i * stride] =
+            decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins, (int)a.F, t0 + i);
 }
 
 // Tile shapes the autotuner chooses from: (workgroup slots, LDS window bytes).
@@ -596,6 +397,12 @@ struct CacheTile {
 constexpr CacheTile CACHE_TILES[] = {{64, 8192}, {64, 16384}, {128, 16384}, {128, 32768}, {256, 32768}, {256, 49152}};
 constexpr int CACHE_TILE_COUNT = (int)(sizeof(CACHE_TILES) / sizeof(CACHE_TILES[0]));
 constexpr int CACHE_TILE_DEFAULT = 2;
+// Up to this many frames per launch the exact per-slot kernel beats staging
+// keyframe streams through LDS (a stream is read whole to be swept).
+#ifndef VBT_CACHE_EXACT_MAX_FRAMES
+#define VBT_CACHE_EXACT_MAX_FRAMES 2
+#endif
+constexpr int CACHE_EXACT_MAX_FRAMES = VBT_CACHE_EXACT_MAX_FRAMES;
 
 template <int LAYOUT>
 void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* groupLeaf, int t0, int nt, float* out,
@@ -1128,13 +935,21 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     if (a->h.profileType == 2) {
         k_sci_leaf<false><<<dim3(a->A, (uint32_t)nt), 512, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
     } else if (layout == VBT_CACHE_PACKED) {
-        launch_cache_render<VBT_CACHE_PACKED>(a->cacheTile, ad, a->d_blockLeaf, t0, nt, d_out, a->controlsPerFrame,
-                                              a->d_err, s);
+        if (nt <= CACHE_EXACT_MAX_FRAMES)
+            k_cache_exact<VBT_CACHE_PACKED><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
+                ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        else
+            launch_cache_render<VBT_CACHE_PACKED>(a->cacheTile, ad, a->d_blockLeaf, t0, nt, d_out,
+                                                  a->controlsPerFrame, a->d_err, s);
     } else {
         const uint64_t n = (uint64_t)a->A * 576u * (uint64_t)nt;
         k_fill<<<(uint32_t)std::min<uint64_t>(blocks_for(n, 256), 65535), 256, 0, s>>>(d_out, n, 0.0f);
-        launch_cache_render<VBT_CACHE_STRIDE576>(a->cacheTile, ad, a->d_blockLeaf, t0, nt, d_out,
-                                                 a->controlsPerFrame, a->d_err, s);
+        if (nt <= CACHE_EXACT_MAX_FRAMES)
+            k_cache_exact<VBT_CACHE_STRIDE576><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
+                ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        else
+            launch_cache_render<VBT_CACHE_STRIDE576>(a->cacheTile, ad, a->d_blockLeaf, t0, nt, d_out,
+                                                     a->controlsPerFrame, a->d_err, s);
     }
     HIPCHK(hipGetLastError());
     if (a->bad) return report_bad(a, "frame cache");

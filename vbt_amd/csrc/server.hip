@@ -32,7 +32,7 @@
 // on the CUs):
 //   query item   queryItem point queries of one request, 8 lanes per query
 //   cache item   cacheItem packed-cache slots of one render frame
-//   sci item     one scientific leaf (its voxels) of one frame
+//   sci item     four scientific leaves (their voxels) of one frame
 #include "vbt_format.h"
 #include "vbt_internal.h"
 #include "vbt_kernels.h"
@@ -64,9 +64,14 @@ namespace {
 
 constexpr uint32_t RING = 65536;  // requests per device-loop launch
 constexpr uint32_t MAX_ASSETS = 32;
-constexpr uint32_t SERVE_BLOCK = 256;
+#ifndef VBT_SERVE_BLOCK
+#define VBT_SERVE_BLOCK 256
+#endif
+constexpr uint32_t SERVE_BLOCK = VBT_SERVE_BLOCK;  // lanes per workgroup
 constexpr uint32_t QUERY_PASS = SERVE_BLOCK / 8u;  // queries per pass of a workgroup
 constexpr uint32_t QUERY_CHUNK = 512;              // queries staged in LDS at once
+constexpr uint32_t SCI_LEAVES = SERVE_BLOCK / 64u; // scientific leaves per item (one control per lane)
+constexpr uint32_t SBUF_WORDS = QUERY_CHUNK * 4u;  // >= SCI_LEAVES * 64 controls
 
 enum : uint32_t { REQ_QUERY = 0, REQ_CACHE = 1 };
 
@@ -221,8 +226,9 @@ k_serve(ServeArgs A)
     }
     __shared__ uint32_t sCmd, sR, sK, sErr;
     __shared__ RingEntry sE;
-    __shared__ float ctrl[64];
-    __shared__ uint32_t sQ[QUERY_CHUNK * 4u];
+    __shared__ uint32_t sBuf[SBUF_WORDS];  // query chunk or scientific controls
+    uint32_t* const sQ = sBuf;
+    float* const ctrl = (float*)sBuf;
     const uint32_t tid = threadIdx.x;
     uint32_t haveR = 0xFFFFFFFFu;  // request whose entry sits in sE (thread 0's view)
     uint64_t waitStart = 0;
@@ -335,33 +341,57 @@ k_serve(ServeArgs A)
                 __syncthreads();
             }
         } else if (a.profile != 2u) {
+            // cacheItem packed-cache slots of one frame, each by the reference
+            // bracket search (one frame needs ~3 keyframe records per slot,
+            // far fewer bytes than staging the slot's whole keyframe stream);
+            // four independent slots per lane at a time for memory parallelism
             float* out = (float*)E.out;
             const uint64_t first = (uint64_t)k * A.cacheItem;
             const uint64_t endS = min<uint64_t>(a.controlsPerFrame, first + A.cacheItem);
-            for (uint64_t g = first + tid; g < endS; g += SERVE_BLOCK) {
-                int32_t st;
-                out[g] = cache_slot_exact(a, g, E.t, st);
-                err |= st != 0;
-            }
-        } else {
-            // one scientific leaf: 64 DCT controls in LDS, then its voxels
-            float* out = (float*)E.out;
-            const DevLeaf& d = a.leaves[k];
-            const bool bad = (d.status & STATUS_MASK) != LEAF_OK;
-            const SciHeader h = sci_header(d.sciHeader);
-            const Bytes b{a.payload + (size_t)d.word * 4u};
-            if (!bad && tid < 64u) ctrl[tid] = sci_control(a, b, h, tid, E.t);
-            __syncthreads();
-            for (uint32_t v = tid; v < 512u; v += SERVE_BLOCK) {
-                const uint32_t lx = v & 7u, ly = (v >> 3) & 7u, lz = v >> 6;
-                if (lx < d.lw && ly < d.lh && lz < d.ld) {
-                    const uint64_t dst = a.ctrl[k] + (lz * d.lh + ly) * d.lw + lx;
-                    out[dst] = bad ? __builtin_nanf("")
-                                   : sci_coarse_sample(ctrl, lx, ly, lz) +
-                                         sci_residual(a, b, h, lx, ly, lz, (uint32_t)E.t);
+            for (uint64_t g0 = first + tid; g0 < endS; g0 += 4u * SERVE_BLOCK) {
+                float v[4];
+                int32_t st[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint64_t g = g0 + (uint64_t)j * SERVE_BLOCK;
+                    v[j] = g < endS ? cache_slot_exact(a, g, E.t, st[j]) : 0.0f;
+                    if (g >= endS) st[j] = 0;
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint64_t g = g0 + (uint64_t)j * SERVE_BLOCK;
+                    if (g < endS) out[g] = v[j];
+                    err |= st[j] != 0;
                 }
             }
-            err |= bad;
+        } else {
+            // SCI_LEAVES scientific leaves: one DCT control per lane into LDS,
+            // then the leaves' voxels across the workgroup
+            float* out = (float*)E.out;
+            const uint32_t c0 = k * SCI_LEAVES;
+            const uint32_t cl = c0 + tid / 64u;
+            if (cl < a.A) {
+                const DevLeaf& d = a.leaves[cl];
+                const Bytes b{a.payload + (size_t)d.word * 4u};
+                ctrl[tid] = (d.status & STATUS_MASK) == LEAF_OK ? sci_control(a, b, sci_header(d.sciHeader), tid & 63u, E.t)
+                                                                : 0.0f;
+            }
+            __syncthreads();
+            for (uint32_t v = tid; v < SCI_LEAVES * 512u; v += SERVE_BLOCK) {
+                const uint32_t c = c0 + v / 512u, vox = v & 511u;
+                if (c >= a.A) break;
+                const DevLeaf& d = a.leaves[c];
+                const uint32_t lx = vox & 7u, ly = (vox >> 3) & 7u, lz = vox >> 6;
+                if (lx < d.lw && ly < d.lh && lz < d.ld) {
+                    const bool bad = (d.status & STATUS_MASK) != LEAF_OK;
+                    const uint64_t dst = a.ctrl[c] + (lz * d.lh + ly) * d.lw + lx;
+                    const Bytes b{a.payload + (size_t)d.word * 4u};
+                    out[dst] = bad ? __builtin_nanf("")
+                                   : sci_coarse_sample(ctrl + (v / 512u) * 64u, lx, ly, lz) +
+                                         sci_residual(a, b, sci_header(d.sciHeader), lx, ly, lz, (uint32_t)E.t);
+                    err |= bad;
+                }
+            }
         }
         if (err) atomicOr(&sErr, 1u);
         release_agent();  // this lane's outputs reach memory before the count
@@ -726,9 +756,11 @@ int vbt_server_create(const vbt_server_config* c, vbt_server* out)
     if (c) cfg = *c;
     if (cfg.maxInflight == 0 || cfg.maxInflight > RING) return vbt_fail(VBT_ERR_INVALID, "maxInflight must be 1..65536");
     if (cfg.queryItem == 0 || cfg.queryItem % QUERY_PASS || cfg.queryItem > 65536)
-        return vbt_fail(VBT_ERR_INVALID, "queryItem must be a positive multiple of 32, at most 65536");
+        return vbt_fail(VBT_ERR_INVALID, "queryItem must be a positive multiple of " + std::to_string(QUERY_PASS) +
+                                             ", at most 65536");
     if (cfg.cacheItem == 0 || cfg.cacheItem % SERVE_BLOCK || cfg.cacheItem > (1u << 24))
-        return vbt_fail(VBT_ERR_INVALID, "cacheItem must be a positive multiple of 256, at most 2^24");
+        return vbt_fail(VBT_ERR_INVALID, "cacheItem must be a positive multiple of " + std::to_string(SERVE_BLOCK) +
+                                             ", at most 2^24");
     if (cfg.blocksPerCU == 0 || cfg.blocksPerCU > 8) return vbt_fail(VBT_ERR_INVALID, "blocksPerCU must be 1..8");
     if (cfg.idleUs == 0 || cfg.leaseMs == 0 || cfg.leaseMs > 60000 || cfg.idleUs > 10000000)
         return vbt_fail(VBT_ERR_INVALID, "idleUs (at most 10 s) and leaseMs (at most 60000) must be positive");
@@ -921,7 +953,7 @@ int vbt_server_submit_cache(vbt_server s, int assetId, int t, uint64_t* requestI
         const uint64_t cpf = s->assetCpf[assetId];
         if (cpf > 0xFFFFFFFFull) return vbt_fail(VBT_ERR_INVALID, "frame cache too large for one request");
         r->n = (uint32_t)cpf;
-        r->items = s->assetProfile[assetId] == 2 ? s->assetActive[assetId]
+        r->items = s->assetProfile[assetId] == 2 ? (s->assetActive[assetId] + SCI_LEAVES - 1u) / SCI_LEAVES
                                                  : (uint32_t)((cpf + s->cfg.cacheItem - 1u) / s->cfg.cacheItem);
     }
     r->type = REQ_CACHE;
