@@ -47,17 +47,17 @@ typedef struct {
     uint8_t fineCodec;
     uint8_t coarseResolution;
     uint8_t fineResolution;
-} vbt_header_fields;         /* RenderTemporalPackedHeaderFields, render_temporal_payload.h:104-110 */
+} vbt_header_fields;         /* RenderTemporalPackedHeaderFields, render_temporal_payload.h:21-27 */
 
 typedef struct {
     uint16_t coarseKeyframeCount;
     uint16_t fineKeyframeCount;
-} vbt_leaf_prefix;           /* RenderTemporalPackedLeafPrefix, render_temporal_payload.h:115-118 */
+} vbt_leaf_prefix;           /* RenderTemporalPackedLeafPrefix, render_temporal_payload.h:32-35 */
 
 typedef struct {
     uint32_t descriptorOffset, descriptorBytes, binIndexOffset, binIndexBytes, keyframeOffset, keyframeBytes,
         totalBytes;
-} vbt_stream_layout;         /* RenderTemporalStreamLayout, render_temporal_payload.h:123-131 */
+} vbt_stream_layout;         /* RenderTemporalStreamLayout, render_temporal_payload.h:40-48 */
 
 typedef struct {
     uint32_t headerBytes, prefixOffset, prefixBytes, coarseOffset;
@@ -65,7 +65,7 @@ typedef struct {
     uint32_t fineOffset;
     vbt_stream_layout fine;
     uint32_t totalBytes;
-} vbt_leaf_layout;           /* RenderTemporalPackedLeafLayout, render_temporal_payload.h:133-142 */
+} vbt_leaf_layout;           /* RenderTemporalPackedLeafLayout, render_temporal_payload.h:50-59 */
 
 typedef struct {
     vbt_header_fields header;
@@ -74,12 +74,12 @@ typedef struct {
     uint16_t shellActiveVoxelCount;
     uint32_t shellResidualOffset;
     uint32_t shellResidualBytes;
-} vbt_leaf_view;             /* RenderTemporalPackedLeafView, render_temporal_payload.h:144-151 */
+} vbt_leaf_view;             /* RenderTemporalPackedLeafView, render_temporal_payload.h:61-68 */
 
 typedef struct {
     uint64_t shellMask[8];
     uint16_t groupPrefix[8];
-} vbt_shell_section;         /* RenderTemporalShellOccupancySection, render_temporal_payload.h:170-173 */
+} vbt_shell_section;         /* RenderTemporalShellOccupancySection, render_temporal_payload.h:87-90 */
 
 typedef struct {
     uint16_t keyStart;
@@ -107,31 +107,31 @@ typedef struct {
 
 uint16_t vbt_float_to_half_bits(float value);   /* renderTemporalFloatToHalfBits, render_temporal_decode.h:35 */
 float vbt_half_bits_to_float(uint16_t bits);    /* renderTemporalHalfBitsToFloat, render_temporal_decode.h:36 */
-uint32_t vbt_keyframe_bytes(uint32_t codec);    /* renderTemporalKeyframeBytes, render_temporal_payload.h:186 */
-uint32_t vbt_pack_header(const vbt_header_fields* fields);                /* render_temporal_payload.h:188 */
-void vbt_unpack_header(uint32_t packedHeader, vbt_header_fields* out);   /* render_temporal_payload.h:189 */
-/* makeRenderTemporalLeafPrefix, render_temporal_payload.h:191 (throws above u16 -> VBT_ERR_FORMAT) */
+uint32_t vbt_keyframe_bytes(uint32_t codec);    /* renderTemporalKeyframeBytes, render_temporal_payload.h:103 */
+uint32_t vbt_pack_header(const vbt_header_fields* fields);                /* render_temporal_payload.h:105 */
+void vbt_unpack_header(uint32_t packedHeader, vbt_header_fields* out);   /* render_temporal_payload.h:106 */
+/* makeRenderTemporalLeafPrefix, render_temporal_payload.h:108 (throws above u16 -> VBT_ERR_FORMAT) */
 int vbt_make_leaf_prefix(uint32_t mode, uint32_t coarseKeyframeCount, uint32_t fineKeyframeCount,
                          vbt_leaf_prefix* out);
-/* validateRenderTemporalLeafPrefix, render_temporal_payload.h:195 (options reduced to timeBinCount) */
+/* validateRenderTemporalLeafPrefix, render_temporal_payload.h:112 (options reduced to timeBinCount) */
 int vbt_validate_leaf_prefix(const vbt_leaf_prefix* prefix, const vbt_header_fields* header, int timeBinCount);
-/* computeRenderTemporalLeafLayout, render_temporal_payload.h:199 */
+/* computeRenderTemporalLeafLayout, render_temporal_payload.h:116 */
 void vbt_compute_leaf_layout(const vbt_header_fields* header, const vbt_leaf_prefix* prefix, int alignTo4Bytes,
                              vbt_leaf_layout* out);
-/* buildRenderTemporalPackedLeafBytes (5-arg), render_temporal_payload.h:203.  Writes at most outCap bytes;
+/* buildRenderTemporalPackedLeafBytes (5-arg), render_temporal_payload.h:120.  Writes at most outCap bytes;
  * *outLen receives the full size (call with out = NULL to size). */
 int vbt_build_leaf_bytes(const vbt_header_fields* header, const vbt_leaf_prefix* prefix, const uint8_t* coarse,
                          size_t coarseLen, const uint8_t* fine, size_t fineLen, int alignTo4Bytes, uint8_t* out,
                          size_t outCap, size_t* outLen);
-/* buildRenderTemporalPackedLeafBytes (6-arg, Mode2), render_temporal_payload.h:211 */
+/* buildRenderTemporalPackedLeafBytes (6-arg, Mode2), render_temporal_payload.h:128 */
 int vbt_build_shell_leaf_bytes(const vbt_header_fields* header, const vbt_leaf_prefix* prefix, const uint8_t* coarse,
                                size_t coarseLen, const uint8_t* occupancy, size_t occupancyLen,
                                const uint8_t* residual, size_t residualLen, int alignTo4Bytes, uint8_t* out,
                                size_t outCap, size_t* outLen);
-void vbt_pack_shell_section(const vbt_shell_section* s, uint8_t out[80]);        /* render_temporal_payload.h:218 */
-void vbt_unpack_shell_section(const uint8_t* data, uint64_t size, vbt_shell_section* out); /* :219 */
-uint16_t vbt_shell_active_count(const vbt_shell_section* s);                      /* :220 */
-/* parseRenderTemporalPackedLeaf, render_temporal_payload.h:222 */
+void vbt_pack_shell_section(const vbt_shell_section* s, uint8_t out[80]);        /* render_temporal_payload.h:135 */
+void vbt_unpack_shell_section(const uint8_t* data, uint64_t size, vbt_shell_section* out); /* :136 */
+uint16_t vbt_shell_active_count(const vbt_shell_section* s);                      /* :137 */
+/* parseRenderTemporalPackedLeaf, render_temporal_payload.h:139 */
 int vbt_parse_leaf(const uint8_t* leafBase, size_t leafBytes, int alignTo4Bytes, vbt_leaf_view* out);
 /* validateRenderTemporalControlStreamData, render_temporal_decode.h:38 -> 1 valid, 0 invalid */
 int vbt_validate_control_stream(const vbt_stream_data* stream);
@@ -189,7 +189,7 @@ typedef struct {
         profileType, coarseAcScaleCount;
 } vbt_file_header;           /* VbtFileHeader, render/src/vbt_file.h:10-23 */
 
-/* loadVbtFile's validation (render/src/vbt_file.cpp:174-275) over an in-memory
+/* loadVbtFile's validation (render/src/vbt_file.cpp:39-105, 109-210) over an in-memory
  * image: same checks, same messages. */
 int vbt_validate_file(const void* bytes, uint64_t size, vbt_file_header* out);
 /* leafIndexForVoxel, render/src/vbt_file.h:63 */

@@ -118,7 +118,7 @@ void* vbtref_load_file(const char* path)
 
 void vbtref_free_file(void* h) { delete static_cast<vbt::render::VbtFile*>(h); }
 
-// render/src/scientific_decode.cpp:349
+// render/src/scientific_decode.cpp:335
 int vbtref_sci_value(void* h, uint32_t x, uint32_t y, uint32_t z, uint32_t t, float* out)
 {
     try {
@@ -131,7 +131,7 @@ int vbtref_sci_value(void* h, uint32_t x, uint32_t y, uint32_t z, uint32_t t, fl
     }
 }
 
-// render/src/scientific_decode.cpp:375
+// render/src/scientific_decode.cpp:361
 int vbtref_sci_frame(void* h, uint32_t t, float* out)
 {
     try {

@@ -73,7 +73,7 @@ static uint64_t rd64(const uint8_t* p) { return (uint64_t)rd32(p) | ((uint64_t)r
 static int popcount64(uint64_t w)
 {
     int c = 0;
-    for (; w != 0; w &= (w - 1)) ++c; /* same loop as payload.cpp:275-277 */
+    for (; w != 0; w &= (w - 1)) ++c; /* same loop as render_temporal_payload.cpp:275-277 */
     return c;
 }
 
@@ -159,7 +159,7 @@ void vbto_unpack_header(uint32_t p, vbto_header* o)
 
 static uint32_t control_count_for_res(uint32_t r) { return r == 0 ? 0u : (uint16_t)(r * r * r); }
 
-/* payload.cpp:62-78 */
+/* render_temporal_payload.cpp:62-78 */
 static void stream_layout(uint32_t start, uint32_t controlCount, uint32_t bins, uint32_t keys,
                           uint32_t codec, int align, vbto_stream_layout* o)
 {
@@ -174,7 +174,7 @@ static void stream_layout(uint32_t start, uint32_t controlCount, uint32_t bins, 
     o->totalBytes = o->descriptorBytes + o->binIndexBytes + o->keyframeBytes;
 }
 
-/* payload.cpp:143-197 */
+/* render_temporal_payload.cpp:143-197 */
 void vbto_compute_leaf_layout(const vbto_header* h, uint16_t ckc, uint16_t fkc, int align,
                               vbto_leaf_layout* o)
 {
@@ -213,7 +213,7 @@ void vbto_compute_leaf_layout(const vbto_header* h, uint16_t ckc, uint16_t fkc, 
     }
 }
 
-/* payload.cpp:324-344 */
+/* render_temporal_payload.cpp:324-344 */
 void vbto_unpack_shell_section(const uint8_t* d, uint64_t size, vbto_shell_section* s)
 {
     memset(s, 0, sizeof(*s));
@@ -229,7 +229,7 @@ uint16_t vbto_shell_active_count(const vbto_shell_section* s)
     return c;
 }
 
-/* payload.cpp:234-311 */
+/* render_temporal_payload.cpp:234-311 */
 int vbto_parse_leaf(const uint8_t* leaf, size_t leafBytes, int align, vbto_leaf_view* o)
 {
     memset(o, 0, sizeof(*o));
@@ -298,7 +298,7 @@ static float key_value(const uint8_t* keyBase, uint32_t codec, uint32_t idx)
     return vbto_half_bits_to_float(rd16(keyBase + off + 1));
 }
 
-/* decodeFromSequence, decode.cpp:79-131.  The caller guarantees (as
+/* decodeFromSequence, render_temporal_decode.cpp:79-131.  The caller guarantees (as
  * decodeRenderTemporalControlValue does) keyStart + keyCount <= keyframeCount,
  * so every read stays inside the keyframe section. */
 static float decode_from_sequence(const uint8_t* keyBase, uint32_t codec, uint32_t keyStart,
@@ -392,7 +392,7 @@ int vbto_decode_control_value(const uint8_t* streamBase, uint32_t codec, uint16_
     return VBTO_OK;
 }
 
-/* decode.cpp:358-398 */
+/* render_temporal_decode.cpp:358-398 */
 int vbto_decode_leaf_control_value(const uint8_t* leaf, const vbto_leaf_view* v, int useFine, int align,
                                    int totalFrames, uint16_t ci, int t, float* out, vbto_trace* tr)
 {
@@ -410,7 +410,7 @@ int vbto_decode_leaf_control_value(const uint8_t* leaf, const vbto_leaf_view* v,
                                      ci, t, out, tr);
 }
 
-/* decode.cpp:404-450 */
+/* render_temporal_decode.cpp:404-450 */
 int vbto_decode_shell_voxel_value(const uint8_t* leaf, const vbto_leaf_view* v, int align, int totalFrames,
                                   uint16_t voxel, int t, float* out, uint32_t* descOut)
 {
@@ -479,7 +479,7 @@ int vbto_load_file_memory(const uint8_t* bytes, uint64_t size, vbto_file* o)
     return VBTO_OK;
 }
 
-/* vbt_file.cpp:277-283 */
+/* render/src/vbt_file.cpp:212-218 */
 uint32_t vbto_read_packed_header_word(const vbto_file* f, uint32_t leaf)
 {
     if (leaf >= f->header.leafCount) return 0;
@@ -488,7 +488,7 @@ uint32_t vbto_read_packed_header_word(const vbto_file* f, uint32_t leaf)
     return f->payloadWords[w];
 }
 
-/* vbt_file.cpp:315-327 */
+/* render/src/vbt_file.cpp:250-262 */
 uint32_t vbto_leaf_index_for_voxel(const vbto_file_header* h, uint32_t x, uint32_t y, uint32_t z)
 {
     if (!h->width || !h->height || !h->depth || !h->leafSize) return 0;
@@ -624,8 +624,8 @@ int vbto_decode_render_leaf_frame(const vbto_file* f, uint32_t leaf, int t, cons
 }
 
 /* ------------------------------------------------------------------------- */
-/* scientific route: render/src/vbt_file.cpp:285-313,                       */
-/* render/src/scientific_decode.cpp:27-423                                   */
+/* scientific route: render/src/vbt_file.cpp:220-248,                       */
+/* render/src/scientific_decode.cpp:13-409                                   */
 /* ------------------------------------------------------------------------- */
 void vbto_decode_sci_header(uint32_t p, vbto_sci_header* d)
 {
@@ -653,7 +653,7 @@ void vbto_decode_sci_header(uint32_t p, vbto_sci_header* d)
     }
 }
 
-/* readPayloadByte/U16 (scientific_decode.cpp:56-73): bounds-checked against
+/* readPayloadByte/U16 (scientific_decode.cpp:42-59): bounds-checked against
  * the whole payload pool like std::vector::at. */
 static int pbyte(const vbto_file* f, uint32_t off, uint32_t* out)
 {
@@ -669,7 +669,7 @@ static int pu16(const vbto_file* f, uint32_t off, uint32_t* out)
     return VBTO_OK;
 }
 
-/* scientific_decode.cpp:86-94 */
+/* scientific_decode.cpp:72-80 */
 float vbto_dct_basis(int N, int index, int k)
 {
     const double kPi = 3.14159265358979323846;
@@ -678,7 +678,7 @@ float vbto_dct_basis(int N, int index, int k)
     return (float)(alpha * cos((kPi / (double)N) * ((double)index + 0.5) * (double)k));
 }
 
-/* decodeCoarseControlAt, scientific_decode.cpp:96-111 */
+/* decodeCoarseControlAt, scientific_decode.cpp:82-97 */
 int vbto_sci_coarse_control(const vbto_file* f, uint32_t base, int keep, int ci, int t, float* out)
 {
     const uint32_t cb = base + (uint32_t)(4 + ci * (keep + 1));
@@ -698,7 +698,7 @@ int vbto_sci_coarse_control(const vbto_file* f, uint32_t base, int keep, int ci,
     return VBTO_OK;
 }
 
-/* sampleCoarseAt, scientific_decode.cpp:113-137 */
+/* sampleCoarseAt, scientific_decode.cpp:99-123 */
 static float sci_sample_coarse(const float* ctrl, uint32_t x, uint32_t y, uint32_t z)
 {
     const float fx = ((float)x / 7.0f) * 3.0f, fy = ((float)y / 7.0f) * 3.0f, fz = ((float)z / 7.0f) * 3.0f;
@@ -719,7 +719,7 @@ static float sci_sample_coarse(const float* ctrl, uint32_t x, uint32_t y, uint32
     return c0 * (1.0f - tz) + c1 * tz;
 }
 
-/* decodeBfp, scientific_decode.cpp:75-84 */
+/* decodeBfp, scientific_decode.cpp:61-70 */
 static float decode_bfp(uint32_t q, float scale, int bits)
 {
     if (bits <= 2) {
@@ -756,7 +756,7 @@ static void dense_coords(int R, uint32_t x, int* x0, float* tx)
     *tx = fx - (float)i;
 }
 
-/* decodeDenseValueAt, scientific_decode.cpp:139-189 */
+/* decodeDenseValueAt, scientific_decode.cpp:125-175 */
 static int sci_dense_value(const vbto_file* f, uint32_t base, int keep, int R, int bits, uint32_t x, uint32_t y,
                            uint32_t z, uint32_t t, float* out)
 {
@@ -791,7 +791,7 @@ static int sci_dense_value(const vbto_file* f, uint32_t base, int keep, int R, i
     return VBTO_OK;
 }
 
-/* decodeDenseTemporalBasisAt, scientific_decode.cpp:191-247 */
+/* decodeDenseTemporalBasisAt, scientific_decode.cpp:177-233 */
 static int sci_dense_basis(const vbto_file* f, uint32_t base, int keep, int R, int bits, int tk, uint32_t x,
                            uint32_t y, uint32_t z, uint32_t t, float* out)
 {
@@ -832,7 +832,7 @@ static int sci_dense_basis(const vbto_file* f, uint32_t base, int keep, int R, i
     return VBTO_OK;
 }
 
-/* decodeSparseResidualAt, scientific_decode.cpp:249-309 */
+/* decodeSparseResidualAt, scientific_decode.cpp:235-295 */
 static int sci_sparse(const vbto_file* f, uint32_t base, const vbto_sci_header* d, int keep, uint32_t idx,
                       uint32_t t, float* out)
 {
@@ -883,7 +883,7 @@ static int sci_sparse(const vbto_file* f, uint32_t base, const vbto_sci_header* 
     return VBTO_OK;
 }
 
-/* decodeScientificResidualAt, scientific_decode.cpp:311-345 */
+/* decodeScientificResidualAt, scientific_decode.cpp:297-331 */
 static int sci_residual(const vbto_file* f, uint32_t base, const vbto_sci_header* d, int keep, uint32_t idx,
                         uint32_t lx, uint32_t ly, uint32_t lz, uint32_t t, float* out)
 {
@@ -900,7 +900,7 @@ static int sci_residual(const vbto_file* f, uint32_t base, const vbto_sci_header
     }
 }
 
-/* decodeScientificValueAtCpu, scientific_decode.cpp:349-373 */
+/* decodeScientificValueAtCpu, scientific_decode.cpp:335-359 */
 int vbto_sci_value_at(const vbto_file* f, uint32_t x, uint32_t y, uint32_t z, uint32_t t, uint32_t leaf, float* out)
 {
     vbto_sci_header d;
@@ -920,7 +920,7 @@ int vbto_sci_value_at(const vbto_file* f, uint32_t x, uint32_t y, uint32_t z, ui
     return VBTO_OK;
 }
 
-/* one leaf of reconstructScientificFrameCpu, scientific_decode.cpp:375-423 */
+/* one leaf of reconstructScientificFrameCpu, render/src/scientific_decode.cpp:361-409 */
 int vbto_sci_leaf_frame(const vbto_file* f, uint32_t leaf, uint32_t t, float* out)
 {
     int bx, by, bz, lw, lh, ld;

@@ -47,7 +47,7 @@ typedef struct {
 
 uint32_t vbto_pack_header(const vbto_header* h);
 void vbto_unpack_header(uint32_t packed, vbto_header* out);
-uint32_t vbto_keyframe_bytes(uint32_t codec); /* payload.cpp:32-41 */
+uint32_t vbto_keyframe_bytes(uint32_t codec); /* render_temporal_payload.cpp:32-41 */
 
 /* ---- layout  (src/render_temporal_payload.cpp:43-78, 143-197) ---- */
 typedef struct {
@@ -183,14 +183,14 @@ int vbto_frame_cache_576(const vbto_file* f, int frameIndex, float* out);
 /* Packed cache: out[ctrlOffset[compact] + slot] with the meaningful counts only. */
 int vbto_frame_cache_packed(const vbto_file* f, int frameIndex, const uint64_t* ctrlOffset, float* out);
 
-/* ---- scientific route  (render/src/vbt_file.cpp:285-313, render/src/scientific_decode.cpp) ---- */
+/* ---- scientific route  (render/src/vbt_file.cpp:220-248, render/src/scientific_decode.cpp) ---- */
 typedef struct {
     uint32_t mode, coarseKeepMinus1, coarseQuantClass, tierId, framesPerBinMinus1, eventQuantCode,
         sparseSubtype, packedEventCount, denseQuantBits, denseSubtype, denseTemporalKeepMinus1;
 } vbto_sci_header;
 
 void vbto_decode_sci_header(uint32_t packed, vbto_sci_header* out);
-float vbto_dct_basis(int totalLength, int index, int k); /* scientific_decode.cpp:86-94 */
+float vbto_dct_basis(int totalLength, int index, int k); /* scientific_decode.cpp:72-80 */
 int vbto_sci_coarse_control(const vbto_file* f, uint32_t payloadByteBase, int keep, int controlIndex,
                             int timeIndex, float* out);
 int vbto_sci_value_at(const vbto_file* f, uint32_t x, uint32_t y, uint32_t z, uint32_t t,

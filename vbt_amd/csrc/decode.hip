@@ -814,7 +814,7 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
     if (h.coarseAcScaleCount)
         TRY(hipMemcpyAsync(a->d_scales, base + scaleOff, 4ull * h.coarseAcScaleCount, hipMemcpyHostToDevice, s));
     if (h.profileType == 2) {
-        // dctBasis table (render/src/scientific_decode.cpp:86-94): computed in double, stored as float
+        // dctBasis table (render/src/scientific_decode.cpp:72-80): computed in double, stored as float
         std::vector<float> basis((size_t)h.frames * 64);
         for (uint32_t t = 0; t < h.frames; ++t)
             for (int k = 0; k < 64; ++k) {

@@ -6,7 +6,7 @@
 //   stream layout         src/render_temporal_payload.cpp:43-78, 143-197
 //   half -> float         src/render_temporal_decode.cpp:167-199
 //   bracket + lerp        src/render_temporal_decode.cpp:79-131
-//   scientific header     render/src/vbt_file.cpp:285-313
+//   scientific header     render/src/vbt_file.cpp:220-248
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -374,7 +374,7 @@ VBT_HD float trilinear(const float s[8], float tx, float ty, float tz)
     return c0 * (1.0f - tz) + c1 * tz;
 }
 
-// ---- scientific route (render/src/vbt_file.cpp:285-313) ----
+// ---- scientific route (render/src/vbt_file.cpp:220-248) ----
 struct SciHeader {
     uint32_t mode, keep, tier, fpb, count, bits, subtype, tk;
 };
@@ -396,7 +396,7 @@ VBT_HD SciHeader sci_header(uint32_t p)
     return h;
 }
 
-// sampleCoarseAt's lattice (render/src/scientific_decode.cpp:113-137):
+// sampleCoarseAt's lattice (render/src/scientific_decode.cpp:99-123):
 // f = (x / 7) * 3, i0 = clamp(floor(f), 0, 2)
 VBT_HD void sci_coarse_coord(uint32_t x, int& i0, float& t)
 {
@@ -407,7 +407,7 @@ VBT_HD void sci_coarse_coord(uint32_t x, int& i0, float& t)
     t = f - (float)i;
 }
 
-// dense lattice of decodeDenseValueAt (render/src/scientific_decode.cpp:155-166)
+// dense lattice of decodeDenseValueAt (render/src/scientific_decode.cpp:141-152)
 VBT_HD void sci_dense_coord(uint32_t x, int R, int& i0, float& t)
 {
     const float f = ((float)x / 7.0f) * (float)(R - 1);
@@ -418,7 +418,7 @@ VBT_HD void sci_dense_coord(uint32_t x, int R, int& i0, float& t)
     t = f - (float)i;
 }
 
-// decodeBfp (render/src/scientific_decode.cpp:75-84)
+// decodeBfp (render/src/scientific_decode.cpp:61-70)
 VBT_HD float decode_bfp(uint32_t q, float scale, int bits)
 {
     if (bits <= 2) {
