@@ -195,6 +195,14 @@ int vbt_validate_file(const void* bytes, uint64_t size, vbt_file_header* out);
 /* leafIndexForVoxel, render/src/vbt_file.h:63 */
 uint32_t vbt_leaf_index_for_voxel(const vbt_file_header* h, uint32_t x, uint32_t y, uint32_t z);
 
+typedef struct {
+    uint32_t mode; /* 0 CoarseOnly, 1 SparseEvents, 2 DenseGrid3, 3 DenseGrid4 */
+    uint32_t coarseKeepMinus1, coarseQuantClass, tierId, framesPerBinMinus1, eventQuantCode, sparseSubtype,
+        packedEventCount, denseQuantBits, denseSubtype, denseTemporalKeepMinus1;
+} vbt_scientific_header;     /* ScientificHeaderV2, render/src/vbt_file.h:39-51 */
+/* decodeScientificHeaderV2, render/src/vbt_file.h:62 */
+void vbt_decode_scientific_header(uint32_t packedHeader, vbt_scientific_header* out);
+
 typedef struct vbt_asset_s* vbt_asset;
 
 /* Validate (as loadVbtFile), upload the offset table, payload pool and AC
@@ -288,9 +296,11 @@ int vbt_query_points_plain(vbt_asset a, const uint32_t* d_queries, uint32_t n, c
  * allow, and appends them to a request ring in host-coherent pinned memory.
  * On the device a persistent loop runs: one scheduler wave polls the ring and
  * publishes new requests to device memory; every other workgroup claims work
- * items (a chunk of queries, a tile of cache slots, one scientific leaf) from
- * the oldest request with unclaimed items, so requests that arrive while
- * others run start on the CUs as soon as workgroups free up.  The workgroup
+ * items (a chunk of queries, a range of cache slots, four scientific leaves)
+ * from the oldest request of a class with unclaimed items -- point-query
+ * batches first in seven of eight workgroups, frame builds first in the
+ * eighth -- so requests that arrive while others run start on the CUs as soon
+ * as workgroups free up.  The workgroup
  * finishing a request's last item writes its completion flag to host memory.
  * The loop exits after idleUs without new requests or after leaseMs, and the
  * dispatcher relaunches it when work arrives; every device wait is bounded by

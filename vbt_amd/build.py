@@ -5,6 +5,7 @@
   vbt_amd/libvbt_hip.so    HIP decode library (gfx950), C-ABI in include/vbt_hip.h
   vbt_amd/libvbt_synth.so  synthetic fields + asset encoder (host C++, OpenMP)
   vbt_amd/libvbt_loadgen.so  open-loop load generator over libvbt_hip.so (host C++)
+  tests/native/shim_check  consumer of the C++ drop-in headers (test program)
   oracle/liboracle.so      CPU oracle (test infrastructure)
 """
 from __future__ import annotations
@@ -97,6 +98,18 @@ def build_loadgen():
     return out
 
 
+def build_shim_check():
+    """tests/native/shim_check: a consumer of the C++ drop-in headers
+    (include/vbt/*.h), used by tests/test_cpp_shim.py."""
+    out = os.path.join(ROOT, "tests", "native", "shim_check")
+    tmp = out + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-o", tmp,
+          os.path.join(ROOT, "tests", "native", "shim_check.cpp"), f"-L{HERE}", "-lvbt_hip", f"-L{ROCM}/lib",
+          "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../vbt_amd", f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle")])
     if os.path.isdir("/root/reference"):
@@ -109,10 +122,11 @@ def build_all():
     build_oracle()
     build_hip()
     build_loadgen()
+    build_shim_check()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "hip": build_hip, "synth": build_synth, "oracle": build_oracle,
-         "loadgen": build_loadgen}[w]()
+         "loadgen": build_loadgen, "shim": build_shim_check}[w]()

@@ -402,6 +402,26 @@ uint32_t vbt_leaf_index_for_voxel(const vbt_file_header* h, uint32_t x, uint32_t
     return (bz * ly + by) * lx + bx;
 }
 
+void vbt_decode_scientific_header(uint32_t p, vbt_scientific_header* o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->mode = p & 3u;
+    o->coarseKeepMinus1 = (p >> 2) & 0x3Fu;
+    if (o->mode == 0u) {
+        o->coarseQuantClass = (p >> 8) & 7u;
+    } else if (o->mode == 1u) {
+        o->tierId = (p >> 8) & 3u;
+        o->framesPerBinMinus1 = (p >> 10) & 0xFu;
+        o->eventQuantCode = (p >> 14) & 3u;
+        o->packedEventCount = (p >> 16) & 0x0FFFu;
+        o->sparseSubtype = (p >> 28) & 0xFu;
+    } else {
+        o->denseQuantBits = (p >> 8) & 0xFu;
+        o->denseSubtype = (p >> 12) & 7u;
+        o->denseTemporalKeepMinus1 = (p >> 15) & 0xFu;
+    }
+}
+
 const char* vbt_status_message(int32_t s)
 {
     switch (s) {
