@@ -190,6 +190,10 @@ def main():
                     help="assets resident beside the frame-cache workload in the serving run")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-queries", action="store_true")
+    ap.add_argument("--tile", type=int, default=-1, help="frame-cache tile shape index (default: autotune)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the decode shards by leaf or "
@@ -212,8 +216,12 @@ def main():
     info = asset.info
     W, H, D, F = asset.dims
     nt = F
-    best, tile_ms = asset.autotune(nt, reps=2)
     shapes = hip.Asset.tile_shapes()
+    if args.tile >= 0:
+        asset.cache_tile = args.tile
+        best, tile_ms = args.tile, [None] * len(shapes)
+    else:
+        best, tile_ms = asset.autotune(nt, reps=2)
     print(f"[bench] autotune: best tile {shapes[best]} ms per tile {tile_ms}", file=sys.stderr, flush=True)
     out = hip.DeviceArray((nt, info.controlsPerFrame), np.float32, s)
     for _ in range(args.warmup):
@@ -238,8 +246,20 @@ def main():
     print(f"[bench] {args.steps} steps: kernel mean {ms_step:.4f} ms (min {ms.min():.4f}), wall/step "
           f"{wall / args.steps * 1e3:.4f} ms", file=sys.stderr, flush=True)
 
-    qres = query_bench(asset, hip, 65536, 50)
-    small = query_bench(asset, hip, 256, 200)
+    if args.copy_probe_mb:
+        nb = args.copy_probe_mb << 20
+        src = hip.DeviceArray(nb // 4, np.float32, s)
+        dst = hip.DeviceArray(nb // 4, np.float32, s)
+        hip.device_copy(dst, src, nb, s)
+        s.sync()
+        print(f"[bench] copy probe: {nb} bytes read + {nb} bytes written by k_copy", file=sys.stderr, flush=True)
+        src.free()
+        dst.free()
+
+    qres = small = None
+    if not args.no_queries:
+        qres = query_bench(asset, hip, 65536, 50)
+        small = query_bench(asset, hip, 256, 200)
 
     line = {
         "metric": "decoded GB/s + % HBM roofline; batched query p50/p99 us vs plain GPU decode",

@@ -367,6 +367,10 @@ int vbt_device_free(void* d_ptr);
 int vbt_memcpy_h2d(void* d_dst, const void* src, size_t bytes, void* stream);
 int vbt_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream);
 int vbt_memset_d(void* d_dst, int value, size_t bytes, void* stream);
+/* Device-to-device copy by a plain copy kernel (k_copy; 16-byte aligned
+ * pointers and size): the known-size reference used to scale HBM byte
+ * counters when profiling the decode kernels. */
+int vbt_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);
 int vbt_stream_create(void** stream);
 int vbt_stream_destroy(void* stream);
 int vbt_stream_synchronize(void* stream);

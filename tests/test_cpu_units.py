@@ -471,7 +471,7 @@ def simulate_cache_windows(klo, khi, pending, block, window):
 
 
 @pytest.mark.parametrize("block,window", [(64, 8192), (64, 16384), (128, 16384), (128, 32768), (256, 32768),
-                                          (256, 49152)])
+                                          (256, 49152), (64, 4096), (128, 8192), (256, 16384), (64, 12288)])
 def test_cache_window_passes_stay_in_bounds(block, window):
     rng = np.random.default_rng(block * 7 + window)
     for trial in range(400):

@@ -6,7 +6,7 @@
 //   payload  u32[P]     the leaf payload pool, byte-addressed by the kernels
 //   scales   f32[S]     scientific AC scales
 //   compact  u32[L]     page table: rank among active leaves (0xffffffff = EMPTY)
-//   status   u32[L]     per-leaf status (LEAF_* | FLAG_SWEEP)
+//   status   u32[L]     per-leaf status (LEAF_* | FLAG_SWEEP | FLAG_NO_NAN)
 //   leaves   DevLeaf[A] per active leaf decode plan (compact order)
 //   ctrl     u64[A+1]   packed-cache control offsets (prefix sum of C(l))
 //   basis    f32[F*64]  scientific DCT basis table (dctBasis values, host-computed)
@@ -274,12 +274,13 @@ __device__ inline uint64_t sci_read_end(const AssetDev& a, const DevLeaf& d)
 // validation: one workgroup per active leaf
 __global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, uint32_t* status)
 {
-    __shared__ uint32_t sErr, sMono;
+    __shared__ uint32_t sErr, sMono, sNoNan;
     const uint32_t c = blockIdx.x;
     DevLeaf& d = leaves[c];
     if (threadIdx.x == 0) {
         sErr = d.status & STATUS_MASK;
         sMono = 1u;
+        sNoNan = 1u;
     }
     __syncthreads();
     if (a.profile == 2u) {
@@ -300,6 +301,11 @@ __global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, u
         for (int s = 0; s < 3; ++s) {
             if (!streams[s]) continue;
             const StreamRef sr = *streams[s];
+            if (sr.codec == CODEC_FP16)
+                for (uint32_t i = threadIdx.x; i < sr.keys; i += blockDim.x) {
+                    const uint32_t h = b.u16(sr.keyBase + 3u * i + 1u);
+                    if ((h & 0x7C00u) == 0x7C00u && (h & 0x3FFu) != 0u) sNoNan = 0u;
+                }
             for (uint32_t i = threadIdx.x; i < sr.count; i += blockDim.x) {
                 const uint32_t ks = b.u16(sr.base + 4u * i), kc = b.u8(sr.base + 4u * i + 2u);
                 if (kc == 0u) continue;
@@ -321,7 +327,8 @@ __global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, u
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t st = sErr | (sErr == LEAF_OK && sMono ? FLAG_SWEEP : 0u);
+        const uint32_t st =
+            sErr | (sErr == LEAF_OK && sMono ? FLAG_SWEEP : 0u) | (sErr == LEAF_OK && sNoNan ? FLAG_NO_NAN : 0u);
         d.status = st;
         status[d.leaf] = st;
     }
@@ -349,6 +356,7 @@ __global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32
     __shared__ uint32_t sRed[BLOCK / 64];
     __shared__ double rtab[256];
     for (uint32_t i = threadIdx.x; i < 256u; i += BLOCK) rtab[i] = i ? 1.0 / (double)i : 0.0;
+    __syncthreads();
     cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed, rtab);
 }
 
@@ -391,12 +399,15 @@ __global__ void __launch_bounds__(256) k_cache_exact(AssetDev a, int t0, int nt,
 }
 
 // Tile shapes the autotuner chooses from: (workgroup slots, LDS window bytes).
+// A larger window serves more wavefronts in one pass, a smaller one leaves
+// room for more resident waves.
 struct CacheTile {
     uint32_t block, window;
 };
-constexpr CacheTile CACHE_TILES[] = {{64, 8192}, {64, 16384}, {128, 16384}, {128, 32768}, {256, 32768}, {256, 49152}};
+constexpr CacheTile CACHE_TILES[] = {{64, 8192},  {64, 16384}, {128, 16384}, {128, 32768}, {256, 32768},
+                                     {256, 49152}, {64, 4096},  {128, 8192},  {256, 16384}, {64, 12288}};
 constexpr int CACHE_TILE_COUNT = (int)(sizeof(CACHE_TILES) / sizeof(CACHE_TILES[0]));
-constexpr int CACHE_TILE_DEFAULT = 2;
+constexpr int CACHE_TILE_DEFAULT = 0;
 // Up to this many frames per launch the exact per-slot kernel beats staging
 // keyframe streams through LDS (a stream is read whole to be swept).
 #ifndef VBT_CACHE_EXACT_MAX_FRAMES
@@ -419,6 +430,10 @@ void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* groupLeaf
         VBT_TILE(3, 128, 32768)
         VBT_TILE(4, 256, 32768)
         VBT_TILE(5, 256, 49152)
+        VBT_TILE(6, 64, 4096)
+        VBT_TILE(7, 128, 8192)
+        VBT_TILE(8, 256, 16384)
+        VBT_TILE(9, 64, 12288)
 #undef VBT_TILE
     default:
         break;
@@ -652,6 +667,13 @@ __global__ void k_shell_values(const uint8_t* leaf, uint32_t leafBytes, int F, c
     }
     out[i] = v;
     if (status) status[i] = st;
+}
+
+// Plain device copy (16 B per lane per step): the reference point that scales
+// the HBM byte counters of the decode kernels.
+__global__ void __launch_bounds__(256) k_copy(const uint4* src, uint4* dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) dst[i] = src[i];
 }
 
 __global__ void k_fill(float* p, uint64_t n, float v)
@@ -1120,6 +1142,18 @@ int vbt_memcpy_d2h(void* d, const void* s, size_t n, void* stream)
 {
     HIPCHK(hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, S(stream)));
     HIPCHK(hipStreamSynchronize(S(stream)));
+    return VBT_OK;
+}
+int vbt_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream)
+{
+    if (bytes % 16u || ((uintptr_t)d_dst | (uintptr_t)d_src) % 16u)
+        return vbt_fail(VBT_ERR_INVALID, "vbt_device_copy needs 16-byte aligned pointers and sizes");
+    const uint64_t n = bytes / 16u;
+    if (n) {
+        k_copy<<<(uint32_t)std::min<uint64_t>(blocks_for(n, 256), 8192), 256, 0, S(stream)>>>((const uint4*)d_src,
+                                                                                            (uint4*)d_dst, n);
+        HIPCHK(hipGetLastError());
+    }
     return VBT_OK;
 }
 int vbt_memset_d(void* d, int v, size_t n, void* stream)

@@ -9,7 +9,8 @@
 
 namespace vbtk {
 
-constexpr uint32_t FLAG_SWEEP = 1u << 8;  // every control's keyframe frames strictly increase
+constexpr uint32_t FLAG_SWEEP = 1u << 8;   // every control's keyframe frames strictly increase
+constexpr uint32_t FLAG_NO_NAN = 1u << 9;  // no fp16 keyframe value of the leaf is a NaN pattern
 constexpr uint32_t STATUS_MASK = 0xFFu;
 constexpr uint32_t NO_LEAF = 0xffffffffu;
 constexpr uint32_t CACHE_GROUP = 64;      // packed-cache slots per group-table entry
@@ -354,18 +355,21 @@ __device__ inline float sweep_alpha(int num, double rinv) { return (float)((doub
 
 // Branch-free form of half_to_float for the sweep: the hardware conversion is
 // exact for zeros, subnormals, normals and infinities; NaNs take the
-// reference's bit mapping (payload shifted, no quieting).
+// reference's bit mapping (payload shifted, no quieting).  Leaves validated
+// free of NaN patterns (FLAG_NO_NAN) skip the NaN mapping.
+template <bool NANFIX>
 __device__ __forceinline__ float sweep_half(uint32_t h)
 {
     _Float16 x;
     const uint16_t hb = (uint16_t)h;
     __builtin_memcpy(&x, &hb, 2);
     const float f = (float)x;
+    if (!NANFIX) return f;
     const uint32_t nanBits = ((h & 0x8000u) << 16) | 0x7F800000u | ((h & 0x3FFu) << 13);
     return ((h & 0x7C00u) == 0x7C00u && (h & 0x3FFu)) ? __uint_as_float(nanBits) : f;
 }
 
-template <uint32_t KB>
+template <uint32_t KB, bool NANFIX>
 __device__ __forceinline__ void sweep_key(const uint8_t* p, uint32_t i, int& f, float& v)
 {
     if (KB == 8u) {
@@ -375,48 +379,70 @@ __device__ __forceinline__ void sweep_key(const uint8_t* p, uint32_t i, int& f, 
     } else {
         const uint8_t* q = p + i * 3u;
         f = (int)q[0];
-        v = sweep_half((uint32_t)q[1] | ((uint32_t)q[2] << 8));
+        v = sweep_half<NANFIX>((uint32_t)q[1] | ((uint32_t)q[2] << 8));
     }
 }
 
-// Sweep of one control over frames t0..t0+nt-1 (out already offset to the
-// slot; consecutive frames are `stride` floats apart).  `keys` points at the
-// control's first keyframe record in LDS.  Requires strictly increasing key
-// frames.  Invariant after each frame: the next key's frame fn exceeds the
-// frame just produced.  Frames advance by at most one per step, so a single
-// predicated advance per step keeps the invariant (the next key is beyond
-// fn because frames strictly increase); an initial catch-up establishes it
-// for t0.  The loop body is branch-free: the key after next is read every
-// step and selected on advance; rtab[b] = RN64(1/b) for gaps b < 256.
+// The forward sweep of one control (strictly increasing key frames).  State:
+// keys k, k+1 and k+2 (clamped to the last key) and RN64(1 / (f[k+1] - f[k])).
+// Invariant after each frame: the next key's frame fn exceeds the frame just
+// produced.  Frames advance by at most one per step, so a single predicated
+// advance per step keeps the invariant (the key after the next is beyond fn
+// because frames strictly increase); a catch-up establishes it for the first
+// frame.  Each frame's bracket is then the adjacent key pair around it, the
+// bracket the reference's hinted binary search returns for increasing keys.
+// rtab[b] = RN64(1/b) for gaps b < 256.
+struct SweepState {
+    uint32_t k;
+    int fk, fn, fp;
+    float vk, vn, vp;
+    double rinv;
+};
+
 template <uint32_t KB>
-__device__ __forceinline__ void sweep_slot(const uint8_t* keys, uint32_t keyCount, int F, int t0, int nt, float* out,
-                                           uint64_t stride, const double* rtab)
+__device__ __forceinline__ double sweep_recip(int gap, const double* rtab)
 {
-    const uint32_t last = keyCount - 1u;
-    auto recip = [&](int gap) -> double {
-        if (KB == 3u) return rtab[gap & 255];
-        return gap < 256 ? rtab[gap] : 1.0 / (double)gap;
-    };
-    uint32_t k = 0;
-    int fk, fn = 0;
-    float vk, vn = 0.0f;
-    sweep_key<KB>(keys, 0, fk, vk);
-    if (last >= 1u) sweep_key<KB>(keys, 1, fn, vn);
-    // catch up to the first requested frame
-    const int first = clamp_frame(t0, F);
-    while (k < last && fn <= first) {
-        ++k;
-        fk = fn;
-        vk = vn;
-        if (k < last) sweep_key<KB>(keys, k + 1u, fn, vn);
+    if (KB == 3u) return rtab[gap & 255];
+    return gap < 256 ? rtab[gap] : 1.0 / (double)gap;
+}
+
+// State at the first requested frame, from the control's records in global
+// memory (g points at key 0).
+template <uint32_t KB, bool NANFIX>
+__device__ __forceinline__ void sweep_start(const uint8_t* g, uint32_t last, int first, const double* rtab,
+                                            SweepState& s)
+{
+    s.k = 0u;
+    s.fn = 0;
+    s.vn = 0.0f;
+    sweep_key<KB, NANFIX>(g, 0u, s.fk, s.vk);
+    if (last >= 1u) sweep_key<KB, NANFIX>(g, 1u, s.fn, s.vn);
+    while (s.k < last && s.fn <= first) {
+        ++s.k;
+        s.fk = s.fn;
+        s.vk = s.vn;
+        if (s.k < last) sweep_key<KB, NANFIX>(g, s.k + 1u, s.fn, s.vn);
     }
-    double rinv = k < last ? recip(fn - fk) : 0.0;
-    int fp;
-    float vp;
-    sweep_key<KB>(keys, k + 2u <= last ? k + 2u : last, fp, vp);
+    s.rinv = s.k < last ? sweep_recip<KB>(s.fn - s.fk, rtab) : 0.0;
+    sweep_key<KB, NANFIX>(g, s.k + 2u <= last ? s.k + 2u : last, s.fp, s.vp);
+}
+
+// cn frames from tBegin on (out at the first; frames `stride` floats apart).
+// rec holds the control's records r0..: record i at rec + (i - r0) * KB; the
+// sweep reads records up to min(k + 2, last) with k <= r0 + cn.  The loop
+// body is branch-free: the key after next is read every step and selected on
+// advance.
+template <uint32_t KB, bool NANFIX>
+__device__ __forceinline__ void sweep_frames(const uint8_t* rec, uint32_t r0, uint32_t last, SweepState& s, int F,
+                                             int tBegin, int cn, float* out, uint64_t stride, const double* rtab)
+{
     const int fmax = F - 1 > 0 ? F - 1 : 0;
-    for (int i = 0; i < nt; ++i) {
-        const int tt = t0 + i;
+    uint32_t k = s.k;
+    int fk = s.fk, fn = s.fn, fp = s.fp;
+    float vk = s.vk, vn = s.vn, vp = s.vp;
+    double rinv = s.rinv;
+    for (int i = 0; i < cn; ++i) {
+        const int tt = tBegin + i;
         const int frame = tt < fmax ? (tt < 0 ? 0 : tt) : fmax;
         const bool adv = (k < last) & (fn <= frame);
         const int gapNew = fp - fn;
@@ -425,13 +451,21 @@ __device__ __forceinline__ void sweep_slot(const uint8_t* keys, uint32_t keyCoun
         fn = adv ? fp : fn;
         vn = adv ? vp : vn;
         k += adv ? 1u : 0u;
-        const double rNew = recip(gapNew > 0 ? gapNew : 1);
+        const double rNew = sweep_recip<KB>(gapNew > 0 ? gapNew : 1, rtab);
         rinv = adv ? rNew : rinv;
-        sweep_key<KB>(keys, k + 2u <= last ? k + 2u : last, fp, vp);
+        sweep_key<KB, NANFIX>(rec, (k + 2u <= last ? k + 2u : last) - r0, fp, vp);
         const float alpha = sweep_alpha(frame - fk, rinv);
         const float lerp = vk + alpha * (vn - vk);
         out[(uint64_t)i * stride] = (fk >= frame || k == last) ? vk : lerp;
     }
+    s.k = k;
+    s.fk = fk;
+    s.fn = fn;
+    s.fp = fp;
+    s.vk = vk;
+    s.vn = vn;
+    s.vp = vp;
+    s.rinv = rinv;
 }
 
 template <uint32_t NW>
@@ -479,23 +513,22 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
 
 // One thread per (active leaf, cache slot): the thread produces that control
 // for frames t0..t0+nt-1.  The workgroup reads its first leaf from the group
-// table and each thread locates its leaf and stream.  Keyframe records of the
-// workgroup's controls are contiguous in the pool (consecutive controls of a
-// stream store consecutive key ranges; active leaves are in leaf order), so
-// the workgroup stages them into an LDS window of WINDOW bytes with dword
-// loads, in as many passes as the window needs.  A pass serves whole
-// wavefronts whose records fit (so each wavefront runs its frame loop once);
-// a wavefront whose records exceed the window is served lane by lane.
-// Leaves whose every control has strictly increasing keyframe frames use the
-// forward sweep (each frame's bracket is then the adjacent key pair around
-// it, exactly the bracket the reference's hinted binary search returns for
-// increasing keys); other leaves run the reference bracket search per frame
-// from global memory.  BLOCK and WINDOW are the tile shape (the autotuner
-// picks it for the one-shot kernel).
-// One tile: slots tile*BLOCK .. tile*BLOCK+BLOCK-1 of the packed cache, for
-// frames t0..t0+nt-1; the workgroup (BLOCK lanes) calls it together.  lds
-// holds WINDOW bytes, sRed BLOCK/64 words; rtab[b] = RN64(1/b) (rtab[0] = 0)
-// is filled before the first call.
+// table and each thread locates its leaf and stream.  Leaves whose every
+// control has strictly increasing keyframe frames use the forward sweep;
+// other leaves run the reference bracket search per frame from global memory.
+// Keyframe records of consecutive controls are contiguous in the pool
+// (consecutive controls of a stream store consecutive key ranges; active
+// leaves are in leaf order), so the workgroup stages them into an LDS window
+// of WINDOW bytes with coalesced dword loads, in as many passes as the window
+// needs, and the lanes sweep from LDS.  A pass serves whole wavefronts whose
+// records fit (so each such wavefront runs its frame loop once); a wavefront
+// whose records exceed the window is served lane by lane over several passes.
+// (Sweeping such wavefronts straight from global memory instead measured
+// slower: the key read sits on the loop's dependency chain.)
+// Tile: slots tile*BLOCK .. tile*BLOCK+BLOCK-1 of the packed cache; lds holds
+// WINDOW bytes, sRed BLOCK/64 words; rtab[b] = RN64(1/b) (rtab[0] = 0) is
+// filled before the first call.  BLOCK and WINDOW are the tile shape (the
+// autotuner picks it for the one-shot kernel).
 template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
 __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint32_t* groupLeaf, uint32_t tile, int t0,
                                                   int nt, float* out, uint64_t total, uint32_t* errFlag,
@@ -554,12 +587,29 @@ __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint3
                     decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins, F, t0 + i);
         }
     }
-    bool pending = sweep && writes;
-    // A wavefront "fits" when its records fit in one window starting at the
-    // dword holding its first record; then it is served in a single pass.
-    const uint32_t waveLo = wave_min(pending ? kLo : 0xFFFFFFFFu);
-    const uint32_t waveHi = wave_max(pending ? kHi : 0u);
+    const bool sw = sweep && writes;
+    const uint32_t mode = kb == 8u ? 0u : ((status & FLAG_NO_NAN) ? 1u : 2u);
+    const uint32_t last = keyCount - 1u;
+    const int first = clamp_frame(t0, F);
+    auto run = [&](const uint8_t* rec) {
+        SweepState st;
+        if (mode == 0u) {
+            sweep_start<8u, false>(rec, last, first, rtab, st);
+            sweep_frames<8u, false>(rec, 0u, last, st, F, t0, nt, o, stride, rtab);
+        } else if (mode == 1u) {
+            sweep_start<3u, false>(rec, last, first, rtab, st);
+            sweep_frames<3u, false>(rec, 0u, last, st, F, t0, nt, o, stride, rtab);
+        } else {
+            sweep_start<3u, true>(rec, last, first, rtab, st);
+            sweep_frames<3u, true>(rec, 0u, last, st, F, t0, nt, o, stride, rtab);
+        }
+    };
+    // a wavefront "fits" when its records fit in one window starting at the
+    // dword holding its first record
+    const uint32_t waveLo = wave_min(sw ? kLo : 0xFFFFFFFFu);
+    const uint32_t waveHi = wave_max(sw ? kHi : 0u);
     const bool waveFits = waveLo != 0xFFFFFFFFu && waveHi - (waveLo & ~3u) <= WINDOW;
+    bool pending = sw;
     for (;;) {
         const uint32_t lo = block_min<NW>(pending ? kLo : 0xFFFFFFFFu, sRed, lane, wave);
         if (lo == 0xFFFFFFFFu) break;
@@ -570,13 +620,19 @@ __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint3
         const bool mine = pending && ((waveFits ? waveHi <= limit : kHi <= limit) || kLo == lo);
         const uint32_t hi = block_max<NW>(mine ? kHi : 0u, sRed, lane, wave);
         const uint32_t nw = hi > w0 * 4u ? (hi + 3u) / 4u - w0 : 0u;
-        const uint32_t* pw = (const uint32_t*)a.payload;
-        for (uint32_t i = tid; i < nw; i += BLOCK) lds[i] = pw[w0 + i];
+        // staging: eight dword loads in flight per lane before the LDS writes
+        const uint32_t* pw = (const uint32_t*)a.payload + w0;
+        for (uint32_t i0 = tid; i0 < nw; i0 += 8u * BLOCK) {
+            uint32_t v[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8u; ++j) v[j] = i0 + j * BLOCK < nw ? pw[i0 + j * BLOCK] : 0u;
+#pragma unroll
+            for (uint32_t j = 0; j < 8u; ++j)
+                if (i0 + j * BLOCK < nw) lds[i0 + j * BLOCK] = v[j];
+        }
         __syncthreads();
         if (mine) {
-            const uint8_t* p = (const uint8_t*)lds + (kLo - w0 * 4u);
-            if (kb == 8u) sweep_slot<8u>(p, keyCount, F, t0, nt, o, stride, rtab);
-            else sweep_slot<3u>(p, keyCount, F, t0, nt, o, stride, rtab);
+            run((const uint8_t*)lds + (kLo - w0 * 4u));
             pending = false;
         }
         __syncthreads();

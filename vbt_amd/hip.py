@@ -92,6 +92,7 @@ def lib():
         L.vbt_memcpy_h2d.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]
         L.vbt_memcpy_d2h.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]
         L.vbt_memset_d.argtypes = [c_void_p, c_int, ctypes.c_size_t, c_void_p]
+        L.vbt_device_copy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]
         L.vbt_stream_create.argtypes = [ctypes.POINTER(c_void_p)]
         L.vbt_stream_destroy.argtypes = [c_void_p]
         L.vbt_stream_synchronize.argtypes = [c_void_p]
@@ -478,3 +479,8 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def device_copy(dst: DeviceArray, src: DeviceArray, nbytes: int, stream: Stream | None = None):
+    """Plain device copy kernel (the byte-counter reference for profiling)."""
+    check(lib().vbt_device_copy(dst.ptr, src.ptr, nbytes, stream.h if stream else None))
