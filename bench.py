@@ -42,6 +42,19 @@ def algorithmic_bytes(info, nt: int) -> dict:
     return {"read": read, "written": written, "page_table": 4.0 * L, "total": read + written + 4.0 * L}
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the frame-cache kernel for this workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (each scaled by what
+    the counter reads for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh).get(workload)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec.get("traffic_bytes")
+
+
 def cpu_baseline(data: bytes, frames: int, budget_s: float, threads: int) -> dict:
     """Oracle (C restatement of the reference decode, -O2) on the host cores:
     packed frame-cache decode of whole frames, one frame per thread."""
@@ -282,7 +295,7 @@ def main():
                             "autotune_ms": dict(zip([f"{b}x{w}" for b, w in shapes], tile_ms))}},
         "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                      "bytes_per_launch": byt["total"], "kernel": "k_cache_render<0>"},
         "queries": {"random_65536": qres, "random_256": small},
     }
