@@ -143,6 +143,68 @@ def query_bench(asset, hip, n_batch: int, reps: int) -> dict:
     return res
 
 
+def timed(hip, stream, fn, steps: int, warmup: int = 1) -> float:
+    """Mean ms per call of fn() on `stream` (HIP events around `steps` calls)."""
+    for _ in range(warmup):
+        fn()
+    stream.sync()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    stream.sync()
+    return e0.elapsed_ms(e1) / steps
+
+
+def batch_bench(hip, assets: dict, names, steps: int) -> dict:
+    """All frames of each named asset into its packed cache: one replay of
+    the batch graph (persistent work-queue kernel, shape autotuned) against
+    one plain launch per asset (each with its own autotuned tile)."""
+    s = hip.Stream()
+    jobs = []
+    byt = {"read": 0.0, "written": 0.0, "page_table": 0.0, "total": 0.0}
+    for n in names:
+        a = assets[n]
+        F = a.dims[3]
+        jobs.append((a, 0, F, hip.CACHE_PACKED, hip.DeviceArray((F, a.info.controlsPerFrame), np.float32, s)))
+        for k, v in algorithmic_bytes(a.info, F).items():
+            byt[k] += v
+    tiles = {}
+    for a, t0, nt, layout, out in jobs:
+        tiles[a] = a.autotune(nt, reps=1)[0]
+
+    def plain():
+        for a, t0, nt, layout, out in jobs:
+            a.frame_cache_into(out, t0, nt, layout, s)
+
+    plain_ms = timed(hip, s, plain, steps)
+    batch = hip.CacheBatch(jobs, stream=s)
+    best, tune_ms, trials = batch.autotune(reps=2)
+    items, wgs = batch.plan_info()
+    graph_ms = timed(hip, s, batch.launch, steps)
+    direct_ms = timed(hip, s, lambda: batch.launch(graph=False), steps)
+    batch.close()
+    for j in jobs:
+        j[4].free()
+    s.close()
+
+    def rates(ms):
+        return {"ms": round(ms, 4), "decoded_gbs": round(byt["written"] / (ms * 1e-3) / 1e9, 1),
+                "roofline_frac": round(byt["total"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    return {"assets": list(names), "frames": [assets[n].dims[3] for n in names],
+            "controls_per_frame": [assets[n].info.controlsPerFrame for n in names],
+            "bytes_per_step": byt["total"], "written_per_step": byt["written"],
+            "plain_per_asset": dict(rates(plain_ms), tiles=[tiles[assets[n]] for n in names]),
+            "batched_graph": rates(graph_ms), "batched_direct": rates(direct_ms),
+            "speedup_graph_over_plain": round(plain_ms / graph_ms, 3),
+            "shape": best, "items": items, "workgroups": wgs,
+            "autotune_ms": {"persistent/tile/ranges/chunk/order/queues/heavyTile": [
+                [t["persistent"], t["tile"], t["ranges"], t["chunk"], t["order"], t["queues"], t["heavyTile"], ms]
+                for t, ms in trials]}}
+
+
 def serving_bench(hip, loadgen, assets: dict, requests: int, seed: int = 0x5EED0005) -> dict:
     """Serving loop over several resident assets: an open-loop trace of
     point-query batches (log-uniform 1..65536 queries, uniform x,y,z,t) and
@@ -204,6 +266,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-queries", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-ceilings", action="store_true")
+    ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
+                    help="assets of the batched frame-cache build (all frames of each, one graph replay)")
     ap.add_argument("--tile", type=int, default=-1, help="frame-cache tile shape index (default: autotune)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (reference for HBM byte counters)")
@@ -259,6 +325,23 @@ def main():
     print(f"[bench] {args.steps} steps: kernel mean {ms_step:.4f} ms (min {ms.min():.4f}), wall/step "
           f"{wall / args.steps * 1e3:.4f} ms", file=sys.stderr, flush=True)
 
+    # bandwidth ceilings of plain kernels on this device (1 GiB write-only fill,
+    # 512 MiB -> 512 MiB copy): what streaming HBM traffic reaches here
+    ceil = {}
+    if not args.no_ceilings:
+        nb = 1 << 30
+        buf = hip.DeviceArray(nb // 4, np.float32, s)
+        ms_fill = timed(hip, s, lambda: hip.device_fill(buf, 0x3F800000, nb, s), 10)
+        half = nb // 2
+        src = hip.DeviceArray(half // 4, np.float32, s)
+        ms_copy = timed(hip, s, lambda: hip.device_copy(buf, src, half, s), 10)
+        ceil = {"fill_gbs": round(nb / (ms_fill * 1e-3) / 1e9, 1),
+                "copy_gbs": round(2 * half / (ms_copy * 1e-3) / 1e9, 1)}
+        buf.free()
+        src.free()
+        print(f"[bench] ceilings: fill {ceil['fill_gbs']} GB/s, copy {ceil['copy_gbs']} GB/s", file=sys.stderr,
+              flush=True)
+
     if args.copy_probe_mb:
         nb = args.copy_probe_mb << 20
         src = hip.DeviceArray(nb // 4, np.float32, s)
@@ -296,22 +379,37 @@ def main():
         "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
-                     "bytes_per_launch": byt["total"], "kernel": "k_cache_render<0>"},
+                     "bytes_per_launch": byt["total"], "kernel": "k_cache_render<0>",
+                     "device_ceilings_gbs": ceil},
         "queries": {"random_65536": qres, "random_256": small},
     }
-    if not args.no_serve:
-        serve = {args.workload: asset}
-        for n in [x for x in args.serve_assets.split(",") if x]:
+    extra = {args.workload: asset}
+    want = ([] if args.no_batch else [x for x in args.batch_assets.split(",") if x]) + \
+        ([] if args.no_serve else [x for x in args.serve_assets.split(",") if x])
+    for n in want:
+        if n not in extra:
             t0 = time.time()
-            serve[n] = hip.Asset(synth.WORKLOADS[n].encode(threads=min(16, os.cpu_count() or 1)))
-            print(f"[bench] serving asset {n}: encoded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+            extra[n] = hip.Asset(synth.WORKLOADS[n].encode(threads=min(16, os.cpu_count() or 1)))
+            print(f"[bench] asset {n}: encoded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    if not args.no_batch:
+        line["batch_single"] = batch_bench(hip, extra, [args.workload], args.steps)
+        names = [x for x in args.batch_assets.split(",") if x]
+        line["batch"] = batch_bench(hip, extra, names, args.steps)
+        bb = line["batch"]
+        print(f"[bench] batch {names}: graph {bb['batched_graph']['ms']} ms vs plain "
+              f"{bb['plain_per_asset']['ms']} ms ({bb['batched_graph']['decoded_gbs']} GB/s decoded), shape "
+              f"{bb['shape']}; single {line['batch_single']['batched_graph']['ms']} ms vs one-shot "
+              f"{line['batch_single']['plain_per_asset']['ms']} ms", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        serve = {n: extra[n] for n in [args.workload] + [x for x in args.serve_assets.split(",") if x]}
         line["serving"] = serving_bench(hip, loadgen, serve, args.serve_requests)
         sat = line["serving"]["saturation"]
         print(f"[bench] serving: server {sat['server']['qps'] / 1e6:.1f} Mq/s vs plain {sat['plain']['qps'] / 1e6:.1f} "
               f"Mq/s at saturation", file=sys.stderr, flush=True)
-        for n, a in serve.items():
-            if a is not asset:
-                a.close()
+
+    for a in extra.values():
+        if a is not asset:
+            a.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data, F, args.cpu_seconds, min(16, os.cpu_count() or 1))
     asset.close()

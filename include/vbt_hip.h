@@ -257,6 +257,61 @@ int vbt_asset_set_cache_tile(vbt_asset a, int tile);
 int vbt_asset_get_cache_tile(vbt_asset a, int* tile);
 int vbt_asset_autotune(vbt_asset a, int nt, int reps, float* msPerTile, int* best, void* stream);
 
+/* ------------------------------------------------ batched frame caches -- */
+/* Frame-cache builds of several assets as one hipGraph replay: the render
+ * jobs (any mix of Grid4 / Fine6 / Mode2 leaves) as ONE persistent kernel over
+ * a work queue of (job, leaf tile, frame range) items; scientific jobs as
+ * kernel nodes on a concurrent branch of the same graph.  Each job's output is
+ * exactly what vbt_decode_frame_cache(asset, t0, nt, layout, d_out) writes.
+ * No counterpart in the reference (it decodes one frame of one asset per
+ * dispatch, vbt_cache.comp:168-236); the work-queue shape is chosen by
+ * vbt_cache_batch_autotune. */
+typedef struct {
+    vbt_asset asset;
+    int t0, nt;   /* frames [t0, t0+nt) */
+    int layout;   /* VBT_CACHE_PACKED or VBT_CACHE_STRIDE576 (render assets only) */
+    float* d_out; /* device memory, nt * frame size floats */
+} vbt_cache_job;
+
+typedef struct {
+    int tile;    /* leaf-tile shape, 0 .. vbt_cache_tile_count()-1 (slots per workgroup x LDS window) */
+    int ranges;  /* frame ranges per leaf tile: each job's frames split into this many items per tile */
+    int chunk;   /* items a workgroup claims per queue access */
+    int order;   /* 0: job / leaf order; 1: heaviest tiles (keyframe bytes) first */
+    int queues;  /* 1: one queue; 8: one queue per XCD (workgroup i starts on queue i % 8, then steals) */
+    int wgPerCU; /* persistent workgroups per CU; 0: as many as fit */
+    int persistent; /* 1: persistent workgroups claim items from the queues; 0: one workgroup per `chunk`
+                       items in queue order (static) */
+    int heavyTile;  /* static form only: tiles whose keyframe bytes exceed the tile's LDS window run in a
+                       second, concurrent kernel with this tile shape (same slots, larger window); -1: none */
+} vbt_batch_shape;
+
+typedef struct vbt_cache_batch_s* vbt_cache_batch;
+
+void vbt_batch_default_shape(vbt_batch_shape* shape);
+/* Plans the queue for `shape` (NULL: default) and captures the graph.  The
+ * assets and outputs must stay alive and unchanged in size while the batch
+ * exists; every asset must live on the current device. */
+int vbt_cache_batch_create(const vbt_cache_job* jobs, int nJobs, const vbt_batch_shape* shape,
+                           vbt_cache_batch* out);
+int vbt_cache_batch_destroy(vbt_cache_batch b);
+/* One replay of the captured graph on `stream` (all jobs).  Returns
+ * VBT_ERR_FORMAT (after launching) when a job's asset has poisoned leaves, as
+ * vbt_decode_frame_cache does. */
+int vbt_cache_batch_launch(vbt_cache_batch b, void* stream);
+/* The same work as plain launches (no graph), for comparison and tuning. */
+int vbt_cache_batch_launch_direct(vbt_cache_batch b, void* stream);
+int vbt_cache_batch_set_shape(vbt_cache_batch b, const vbt_batch_shape* shape);
+int vbt_cache_batch_get_shape(vbt_cache_batch b, vbt_batch_shape* shape);
+/* Work items of the current plan and persistent workgroups of its launch. */
+int vbt_cache_batch_plan_info(vbt_cache_batch b, uint64_t* items, uint32_t* workgroups);
+/* Times candidate shapes (reps direct launches each, coordinate descent over
+ * kernel form x tile, then ranges, chunk, order, queues, heavy tile), keeps
+ * the fastest and re-captures the graph.  Trials can be read back with vbt_cache_batch_trial. */
+int vbt_cache_batch_autotune(vbt_cache_batch b, int reps, vbt_batch_shape* best, float* bestMs, void* stream);
+int vbt_cache_batch_trial_count(vbt_cache_batch b);
+int vbt_cache_batch_trial(vbt_cache_batch b, int i, vbt_batch_shape* shape, float* ms);
+
 /* --------------------------------------------------- voxels & queries -- */
 enum {
     VBT_SIGN_NONE = 0,        /* render_temporal_vbt_to_vdb default (optional shellEmptyScale) */
@@ -371,6 +426,16 @@ int vbt_memset_d(void* d_dst, int value, size_t bytes, void* stream);
  * pointers and size): the known-size reference used to scale HBM byte
  * counters when profiling the decode kernels. */
 int vbt_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);
+/* Write-only reference: fills bytes (16-byte aligned) with a 32-bit word by a
+ * plain store kernel; the write-bandwidth ceiling next to vbt_device_copy's
+ * copy ceiling, and a clear that a captured graph may contain. */
+int vbt_device_fill(void* d_dst, uint32_t word, size_t bytes, void* stream);
+/* Diagnostic without a counterpart in the reference: counts the lerp weights
+ * (frame - f0) / (f1 - f0), 0 <= frame - f0 < f1 - f0 <= maxGap, for which the
+ * frame-cache sweep's reciprocal-based weight differs from the correctly
+ * rounded float quotient the reference computes (all 2^31 of them for
+ * maxGap = 65535; expected 0).  Synchronises `stream`. */
+int vbt_selftest_weights(uint32_t maxGap, uint64_t* mismatches, void* stream);
 int vbt_stream_create(void** stream);
 int vbt_stream_destroy(void* stream);
 int vbt_stream_synchronize(void* stream);

@@ -92,3 +92,11 @@ def test_error_statuses():
     got, st = hip.decode_control_values(packed, 0, 1, 8, len(s.keys), 20, np.array([0], np.uint16),
                                         np.array([3], np.int32), stream_limit=len(packed) - 4)
     assert st[0] == 10
+
+
+def test_sweep_lerp_weights_for_every_gap():
+    """The frame-cache sweep forms (frame - f0) / (f1 - f0) from a register
+    reciprocal; every weight with gaps up to 65535 (fp32 key frames are 16-bit)
+    must equal the correctly rounded float quotient of the reference."""
+    assert hip.selftest_weights(65535) == 0
+    assert hip.selftest_weights(1) == 0

@@ -354,10 +354,7 @@ __global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32
 {
     __shared__ uint32_t lds[WINDOW / 4];
     __shared__ uint32_t sRed[BLOCK / 64];
-    __shared__ double rtab[256];
-    for (uint32_t i = threadIdx.x; i < 256u; i += BLOCK) rtab[i] = i ? 1.0 / (double)i : 0.0;
-    __syncthreads();
-    cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed, rtab);
+    cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed);
 }
 
 // Few frames: one thread per cache slot, the reference bracket search per
@@ -397,23 +394,6 @@ __global__ void __launch_bounds__(256) k_cache_exact(AssetDev a, int t0, int nt,
         o[(uint64_t)i * stride] =
             decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins, (int)a.F, t0 + i);
 }
-
-// Tile shapes the autotuner chooses from: (workgroup slots, LDS window bytes).
-// A larger window serves more wavefronts in one pass, a smaller one leaves
-// room for more resident waves.
-struct CacheTile {
-    uint32_t block, window;
-};
-constexpr CacheTile CACHE_TILES[] = {{64, 8192},  {64, 16384}, {128, 16384}, {128, 32768}, {256, 32768},
-                                     {256, 49152}, {64, 4096},  {128, 8192},  {256, 16384}, {64, 12288}};
-constexpr int CACHE_TILE_COUNT = (int)(sizeof(CACHE_TILES) / sizeof(CACHE_TILES[0]));
-constexpr int CACHE_TILE_DEFAULT = 0;
-// Up to this many frames per launch the exact per-slot kernel beats staging
-// keyframe streams through LDS (a stream is read whole to be swept).
-#ifndef VBT_CACHE_EXACT_MAX_FRAMES
-#define VBT_CACHE_EXACT_MAX_FRAMES 2
-#endif
-constexpr int CACHE_EXACT_MAX_FRAMES = VBT_CACHE_EXACT_MAX_FRAMES;
 
 template <int LAYOUT>
 void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* groupLeaf, int t0, int nt, float* out,
@@ -676,6 +656,28 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* src, uint4* dst, uint
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) dst[i] = src[i];
 }
 
+__global__ void __launch_bounds__(256) k_fill16(uint4* dst, uint4 v, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) dst[i] = v;
+}
+
+// Every lerp weight the sweep can form: block g-1 checks gap g against the
+// correctly rounded double quotient rounded to float (equal to the correctly
+// rounded float quotient for these operands; see sweep_alpha).
+__global__ void __launch_bounds__(256) k_selftest_weights(uint32_t maxGap, unsigned long long* mismatches)
+{
+    const uint32_t g = blockIdx.x + 1u;
+    if (g > maxGap) return;
+    const double r = sweep_recip((int)g);
+    uint32_t bad = 0;
+    for (uint32_t a = threadIdx.x; a < g; a += 256u) {
+        const float got = sweep_alpha((int)a, r);
+        const float exp = (float)((double)a / (double)g);
+        bad += __float_as_uint(got) != __float_as_uint(exp) ? 1u : 0u;
+    }
+    if (bad) atomicAdd(mismatches, (unsigned long long)bad);
+}
+
 __global__ void k_fill(float* p, uint64_t n, float v)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -779,6 +781,9 @@ vbtk::AssetDev vbt_internal_asset_dev(vbt_asset a) { return a->dev(); }
 uint64_t vbt_internal_asset_bytes(vbt_asset a) { return a->deviceBytes; }
 int vbt_internal_asset_device(vbt_asset a) { return a->device; }
 uint32_t vbt_internal_asset_bad(vbt_asset a) { return a->bad; }
+uint32_t* vbt_internal_asset_err(vbt_asset a) { return a->d_err; }
+int vbt_internal_report_bad(vbt_asset a, const char* what) { return report_bad(a, what); }
+const vbt_file_header* vbt_internal_asset_header(vbt_asset a) { return &a->h; }
 
 extern "C" {
 
@@ -951,6 +956,8 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
         return vbt_fail(VBT_ERR_INVALID, "unknown cache layout");
     if (a->h.profileType == 2 && layout != VBT_CACHE_PACKED)
         return vbt_fail(VBT_ERR_INVALID, "scientific assets support the packed cache only");
+    if (4ull * (layout == VBT_CACHE_PACKED ? a->controlsPerFrame : (uint64_t)a->A * 576u) > 0xFFFFFFFFull)
+        return vbt_fail(VBT_ERR_INVALID, "frame cache: one frame exceeds 4 GiB");
     hipStream_t s = S(stream);
     const AssetDev ad = a->dev();
     if (a->A == 0) return a->bad ? report_bad(a, "frame cache") : VBT_OK;
@@ -1154,6 +1161,37 @@ int vbt_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream)
                                                                                             (uint4*)d_dst, n);
         HIPCHK(hipGetLastError());
     }
+    return VBT_OK;
+}
+int vbt_device_fill(void* d_dst, uint32_t word, size_t bytes, void* stream)
+{
+    if (bytes % 16u || (uintptr_t)d_dst % 16u)
+        return vbt_fail(VBT_ERR_INVALID, "vbt_device_fill needs a 16-byte aligned pointer and size");
+    const uint64_t n = bytes / 16u;
+    if (n) {
+        k_fill16<<<(uint32_t)std::min<uint64_t>(blocks_for(n, 256), 16384), 256, 0, S(stream)>>>(
+            (uint4*)d_dst, make_uint4(word, word, word, word), n);
+        HIPCHK(hipGetLastError());
+    }
+    return VBT_OK;
+}
+int vbt_selftest_weights(uint32_t maxGap, uint64_t* mismatches, void* stream)
+{
+    if (!mismatches || maxGap == 0u || maxGap > 65535u)
+        return vbt_fail(VBT_ERR_INVALID, "vbt_selftest_weights: maxGap must be 1..65535");
+    unsigned long long* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, sizeof(*d)));
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(*d), S(stream));
+    if (e == hipSuccess) {
+        k_selftest_weights<<<maxGap, 256, 0, S(stream)>>>(maxGap, d);
+        e = hipGetLastError();
+    }
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, S(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
+    (void)hipFree(d);
+    HIPCHK(e);
+    *mismatches = h;
     return VBT_OK;
 }
 int vbt_memset_d(void* d, int v, size_t n, void* stream)

@@ -55,11 +55,6 @@
 
 using namespace vbtk;
 
-vbtk::AssetDev vbt_internal_asset_dev(vbt_asset a);
-uint64_t vbt_internal_asset_bytes(vbt_asset a);
-int vbt_internal_asset_device(vbt_asset a);
-uint32_t vbt_internal_asset_bad(vbt_asset a);
-
 namespace {
 
 constexpr uint32_t RING = 65536;  // requests per device-loop launch

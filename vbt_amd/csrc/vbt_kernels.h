@@ -15,6 +15,23 @@ constexpr uint32_t STATUS_MASK = 0xFFu;
 constexpr uint32_t NO_LEAF = 0xffffffffu;
 constexpr uint32_t CACHE_GROUP = 64;      // packed-cache slots per group-table entry
 
+// Tile shapes the autotuner chooses from: (workgroup slots, LDS window bytes).
+// A larger window serves more wavefronts in one pass, a smaller one leaves
+// room for more resident waves.
+struct CacheTile {
+    uint32_t block, window;
+};
+constexpr CacheTile CACHE_TILES[] = {{64, 8192},  {64, 16384}, {128, 16384}, {128, 32768}, {256, 32768},
+                                     {256, 49152}, {64, 4096},  {128, 8192},  {256, 16384}, {64, 12288}};
+constexpr int CACHE_TILE_COUNT = (int)(sizeof(CACHE_TILES) / sizeof(CACHE_TILES[0]));
+constexpr int CACHE_TILE_DEFAULT = 0;
+// Up to this many frames per launch the exact per-slot kernel beats staging
+// keyframe streams through LDS (a stream is read whole to be swept).
+#ifndef VBT_CACHE_EXACT_MAX_FRAMES
+#define VBT_CACHE_EXACT_MAX_FRAMES 2
+#endif
+constexpr int CACHE_EXACT_MAX_FRAMES = VBT_CACHE_EXACT_MAX_FRAMES;
+
 struct DevLeaf {
     uint32_t leaf, word, bytes, status;
     uint32_t ncache, cc;       // cache slots; coarse slots
@@ -335,23 +352,15 @@ __device__ inline float cache_slot_exact(const AssetDev& a, uint64_t g, int t, i
 
 // ------------------------------------------------- packed cache: render tile --
 // Lerp weight of the sweep: alpha = (frame - f0) / (f1 - f0) for integer
-// frames 0 <= frame - f0 < f1 - f0 < 2^16.  With r = RN64(1 / (f1 - f0)),
-// RN32(RN64((frame - f0) * r)) equals the correctly rounded float quotient the
-// reference computes: the double product is within 2^-52 (relative) of the
-// true quotient, while a quotient a/b with b < 2^16 lies at least 2^-41
-// (relative) away from every float rounding boundary, so both round alike.
+// frames 0 <= frame - f0 < f1 - f0 < 2^16, computed as RN32(RN64((frame - f0)
+// * r)) with r within 2^-50 (relative) of 1 / (f1 - f0).  This equals the
+// correctly rounded float quotient the reference computes: the double product
+// is within 2^-49 (relative) of the true quotient, while a quotient a/b with
+// b < 2^16 that is not itself a float lies at least 2^-41 (relative) away from
+// every float rounding boundary (a boundary is a 25-bit odd multiple of a
+// power of two; |a/b - m 2^e| >= 2^e / b), and one that is a float is
+// reproduced exactly.
 __device__ inline float sweep_alpha(int num, double rinv) { return (float)((double)num * rinv); }
-
-// One thread per (active leaf, cache slot); the thread produces that control
-// for frames t0..t0+nt-1.  The workgroup first locates its leaf range in the
-// page table (two binary searches per workgroup) and stages the leaves'
-// payload bytes -- contiguous in the pool because active leaves are in leaf
-// order -- into LDS with dword loads, so keyframe reads hit LDS.  Leaves
-// whose every control has strictly increasing keyframe frames use a forward
-// sweep over the keyframes: each frame's bracket is then the adjacent key pair
-// around it, which is exactly the bracket the reference's hinted binary
-// search returns for increasing keys.  Other leaves run the reference
-// bracket search per frame.
 
 // Branch-free form of half_to_float for the sweep: the hardware conversion is
 // exact for zeros, subnormals, normals and infinities; NaNs take the
@@ -384,14 +393,13 @@ __device__ __forceinline__ void sweep_key(const uint8_t* p, uint32_t i, int& f, 
 }
 
 // The forward sweep of one control (strictly increasing key frames).  State:
-// keys k, k+1 and k+2 (clamped to the last key) and RN64(1 / (f[k+1] - f[k])).
+// keys k, k+1 and k+2 (clamped to the last key) and 1 / (f[k+1] - f[k]).
 // Invariant after each frame: the next key's frame fn exceeds the frame just
 // produced.  Frames advance by at most one per step, so a single predicated
 // advance per step keeps the invariant (the key after the next is beyond fn
 // because frames strictly increase); a catch-up establishes it for the first
 // frame.  Each frame's bracket is then the adjacent key pair around it, the
 // bracket the reference's hinted binary search returns for increasing keys.
-// rtab[b] = RN64(1/b) for gaps b < 256.
 struct SweepState {
     uint32_t k;
     int fk, fn, fp;
@@ -399,18 +407,24 @@ struct SweepState {
     double rinv;
 };
 
-template <uint32_t KB>
-__device__ __forceinline__ double sweep_recip(int gap, const double* rtab)
+// 1 / gap for a key gap 1 <= gap < 2^16 (0 is mapped to 1: such a weight is
+// never used): the hardware double reciprocal refined by two Newton steps,
+// well inside the 2^-50 the weight needs (vbt_selftest_weights checks every
+// gap against a correctly rounded division).
+__device__ __forceinline__ double sweep_recip(int gap)
 {
-    if (KB == 3u) return rtab[gap & 255];
-    return gap < 256 ? rtab[gap] : 1.0 / (double)gap;
+    const double b = (double)(gap > 1 ? gap : 1);
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    return __builtin_fma(r, e, r);
 }
 
-// State at the first requested frame, from the control's records in global
-// memory (g points at key 0).
+// State at the first requested frame from the control's records (g points at
+// key 0): the catch-up walk that establishes the invariant.
 template <uint32_t KB, bool NANFIX>
-__device__ __forceinline__ void sweep_start(const uint8_t* g, uint32_t last, int first, const double* rtab,
-                                            SweepState& s)
+__device__ __forceinline__ void sweep_start(const uint8_t* g, uint32_t last, int first, SweepState& s)
 {
     s.k = 0u;
     s.fn = 0;
@@ -423,49 +437,97 @@ __device__ __forceinline__ void sweep_start(const uint8_t* g, uint32_t last, int
         s.vk = s.vn;
         if (s.k < last) sweep_key<KB, NANFIX>(g, s.k + 1u, s.fn, s.vn);
     }
-    s.rinv = s.k < last ? sweep_recip<KB>(s.fn - s.fk, rtab) : 0.0;
+    s.rinv = s.k < last ? sweep_recip(s.fn - s.fk) : 0.0;
     sweep_key<KB, NANFIX>(g, s.k + 2u <= last ? s.k + 2u : last, s.fp, s.vp);
 }
 
-// cn frames from tBegin on (out at the first; frames `stride` floats apart).
-// rec holds the control's records r0..: record i at rec + (i - r0) * KB; the
-// sweep reads records up to min(k + 2, last) with k <= r0 + cn.  The loop
-// body is branch-free: the key after next is read every step and selected on
-// advance.
+typedef __attribute__((address_space(3))) const uint8_t lds_byte;
+typedef uint16_t u16_any_align __attribute__((aligned(1)));
+
+// Record of a control staged in LDS at byte offset off (32-bit LDS addressing).
 template <uint32_t KB, bool NANFIX>
-__device__ __forceinline__ void sweep_frames(const uint8_t* rec, uint32_t r0, uint32_t last, SweepState& s, int F,
-                                             int tBegin, int cn, float* out, uint64_t stride, const double* rtab)
+__device__ __forceinline__ void lds_key(lds_byte* L, uint32_t off, int& f, float& v)
 {
-    const int fmax = F - 1 > 0 ? F - 1 : 0;
+    if (KB == 8u) {
+        f = (int)((uint32_t)L[off] | ((uint32_t)L[off + 1u] << 8));
+        v = __uint_as_float((uint32_t)L[off + 4u] | ((uint32_t)L[off + 5u] << 8) | ((uint32_t)L[off + 6u] << 16) |
+                            ((uint32_t)L[off + 7u] << 24));
+    } else {
+        f = (int)L[off];
+        v = sweep_half<NANFIX>(*(__attribute__((address_space(3))) const u16_any_align*)(L + off + 1u));
+    }
+}
+
+// Raw buffer descriptor of one frame of the output (base, size in bytes):
+// stores address it with a scalar base and a 32-bit lane offset, and stores
+// at or beyond `bytes` are dropped by the hardware range check.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(char* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, 0x00020000);
+}
+
+// Frames tBegin .. tBegin+cn-1 (all inside [0, F)) of one control from state
+// s at frame tBegin; its records are in LDS at byte offset off (record i at
+// off + i * KB).  Frame i goes to out + i * strideBytes + dstBytes, with out
+// the same for every lane: a buffer store with the frame's base in scalar
+// registers and the lane's 32-bit offset (strideBytes < 2^32, dstBytes <
+// strideBytes; the host checks the frame size).  The state update is
+// branch-free: the key after next (record min(k + 2, last), tracked as a byte
+// offset) is read every step and selected on advance.  Step i issues that LDS
+// read first and produces frame i - 1's output (saved by step i - 1) while
+// the read is in flight, then updates the state for frame i.
+template <uint32_t KB, bool NANFIX>
+__device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t last, const SweepState& s,
+                                             int tBegin, int cn, char* out, uint32_t dstBytes, uint64_t strideBytes)
+{
     uint32_t k = s.k;
-    int fk = s.fk, fn = s.fn, fp = s.fp;
-    float vk = s.vk, vn = s.vn, vp = s.vp;
+    int fk = s.fk, fn = s.fn;
+    float vk = s.vk, vn = s.vn;
+    int fp = s.fp;
+    float vp = s.vp;
     double rinv = s.rinv;
-    for (int i = 0; i < cn; ++i) {
-        const int tt = tBegin + i;
-        const int frame = tt < fmax ? (tt < 0 ? 0 : tt) : fmax;
+    const uint32_t lastOff = off + last * KB;
+    // byte offset of record min(k + 2, last); the record itself is (fp, vp)
+    uint32_t pOff = min(off + (k + 2u) * KB, lastOff);
+    // step i's values for frame i's output, produced in step i + 1
+    int fkS = 0;
+    float vkS = 0.0f, vnS = 0.0f;
+    bool endS = false;
+    double rinvS = 0.0;
+    auto update = [&](int frame) {
         const bool adv = (k < last) & (fn <= frame);
-        const int gapNew = fp - fn;
+        const uint32_t step = adv ? 1u : 0u;
+        const double rNew = sweep_recip(fp - fn);
         fk = adv ? fn : fk;
         vk = adv ? vn : vk;
         fn = adv ? fp : fn;
         vn = adv ? vp : vn;
-        k += adv ? 1u : 0u;
-        const double rNew = sweep_recip<KB>(gapNew > 0 ? gapNew : 1, rtab);
+        k += step;
         rinv = adv ? rNew : rinv;
-        sweep_key<KB, NANFIX>(rec, (k + 2u <= last ? k + 2u : last) - r0, fp, vp);
-        const float alpha = sweep_alpha(frame - fk, rinv);
-        const float lerp = vk + alpha * (vn - vk);
-        out[(uint64_t)i * stride] = (fk >= frame || k == last) ? vk : lerp;
+        pOff = min(pOff + step * KB, lastOff);
+        lds_key<KB, NANFIX>(L, pOff, fp, vp);
+        fkS = fk;
+        vkS = vk;
+        vnS = vn;
+        endS = k == last;
+        rinvS = rinv;
+    };
+    auto output = [&](int frame) {
+        const float alpha = sweep_alpha(frame - fkS, rinvS);
+        const float lerp = vkS + alpha * (vnS - vkS);
+        const float val = (fkS >= frame || endS) ? vkS : lerp;
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(val), frame_rsrc(out, (uint32_t)strideBytes),
+                                              (int)dstBytes, 0, 0);
+        out += strideBytes;
+    };
+    // one basic block per step: the record read at the end of a step stays
+    // there, and the next step's output stage runs while it is in flight
+    update(tBegin);
+    for (int i = 1; i < cn; ++i) {
+        output(tBegin + i - 1);
+        update(tBegin + i);
     }
-    s.k = k;
-    s.fk = fk;
-    s.fn = fn;
-    s.fp = fp;
-    s.vk = vk;
-    s.vn = vn;
-    s.vp = vp;
-    s.rinv = rinv;
+    output(tBegin + cn - 1);
 }
 
 template <uint32_t NW>
@@ -526,13 +588,12 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
 // (Sweeping such wavefronts straight from global memory instead measured
 // slower: the key read sits on the loop's dependency chain.)
 // Tile: slots tile*BLOCK .. tile*BLOCK+BLOCK-1 of the packed cache; lds holds
-// WINDOW bytes, sRed BLOCK/64 words; rtab[b] = RN64(1/b) (rtab[0] = 0) is
-// filled before the first call.  BLOCK and WINDOW are the tile shape (the
+// WINDOW bytes, sRed BLOCK/64 words.  BLOCK and WINDOW are the tile shape (the
 // autotuner picks it for the one-shot kernel).
 template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
 __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint32_t* groupLeaf, uint32_t tile, int t0,
                                                   int nt, float* out, uint64_t total, uint32_t* errFlag,
-                                                  uint32_t* lds, uint32_t* sRed, const double* rtab)
+                                                  uint32_t* lds, uint32_t* sRed)
 {
     constexpr uint32_t NW = BLOCK / 64;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -590,18 +651,25 @@ __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint3
     const bool sw = sweep && writes;
     const uint32_t mode = kb == 8u ? 0u : ((status & FLAG_NO_NAN) ? 1u : 2u);
     const uint32_t last = keyCount - 1u;
-    const int first = clamp_frame(t0, F);
-    auto run = [&](const uint8_t* rec) {
+    const int first = t0;
+    // frames t0 .. t0+nt-1 lie inside [0, F) (checked by the host); slot byte
+    // offsets within a frame fit 32 bits (checked by the host)
+    char* const ob = (char*)out;
+    const uint32_t dstBytes = (uint32_t)dst * 4u;
+    const uint64_t strideBytes = stride * 4u;
+    auto run = [&](uint32_t off) {
+        lds_byte* L = (lds_byte*)lds;
+        const uint8_t* rec = (const uint8_t*)lds + off;
         SweepState st;
         if (mode == 0u) {
-            sweep_start<8u, false>(rec, last, first, rtab, st);
-            sweep_frames<8u, false>(rec, 0u, last, st, F, t0, nt, o, stride, rtab);
+            sweep_start<8u, false>(rec, last, first, st);
+            sweep_frames<8u, false>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
         } else if (mode == 1u) {
-            sweep_start<3u, false>(rec, last, first, rtab, st);
-            sweep_frames<3u, false>(rec, 0u, last, st, F, t0, nt, o, stride, rtab);
+            sweep_start<3u, false>(rec, last, first, st);
+            sweep_frames<3u, false>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
         } else {
-            sweep_start<3u, true>(rec, last, first, rtab, st);
-            sweep_frames<3u, true>(rec, 0u, last, st, F, t0, nt, o, stride, rtab);
+            sweep_start<3u, true>(rec, last, first, st);
+            sweep_frames<3u, true>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
         }
     };
     // a wavefront "fits" when its records fit in one window starting at the
@@ -632,7 +700,7 @@ __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint3
         }
         __syncthreads();
         if (mine) {
-            run((const uint8_t*)lds + (kLo - w0 * 4u));
+            run(kLo - w0 * 4u);
             pending = false;
         }
         __syncthreads();

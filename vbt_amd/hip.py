@@ -70,6 +70,18 @@ class ServerStats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class CacheJob(ctypes.Structure):
+    _fields_ = [("asset", c_void_p), ("t0", c_int), ("nt", c_int), ("layout", c_int), ("d_out", c_void_p)]
+
+
+class BatchShape(ctypes.Structure):
+    _fields_ = [(n, c_int) for n in ("tile", "ranges", "chunk", "order", "queues", "wgPerCU", "persistent",
+                                       "heavyTile")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 def compose(sign_rule=SIGN_NONE, background=0.0, shell_empty_scale=1.0, shell_empty_max=math.nan) -> Compose:
     return Compose(sign_rule, background, shell_empty_scale, shell_empty_max)
 
@@ -93,6 +105,8 @@ def lib():
         L.vbt_memcpy_d2h.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]
         L.vbt_memset_d.argtypes = [c_void_p, c_int, ctypes.c_size_t, c_void_p]
         L.vbt_device_copy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]
+        L.vbt_device_fill.argtypes = [c_void_p, c_u32, ctypes.c_size_t, c_void_p]
+        L.vbt_selftest_weights.argtypes = [c_u32, ctypes.POINTER(c_u64), c_void_p]
         L.vbt_stream_create.argtypes = [ctypes.POINTER(c_void_p)]
         L.vbt_stream_destroy.argtypes = [c_void_p]
         L.vbt_stream_synchronize.argtypes = [c_void_p]
@@ -141,6 +155,21 @@ def lib():
         L.vbt_server_release.argtypes = [c_void_p, c_u64]
         L.vbt_server_stats_get.argtypes = [c_void_p, c_int, ctypes.POINTER(ServerStats)]
         L.vbt_server_reset_stats.argtypes = [c_void_p]
+        L.vbt_batch_default_shape.argtypes = [ctypes.POINTER(BatchShape)]
+        L.vbt_batch_default_shape.restype = None
+        L.vbt_cache_batch_create.argtypes = [ctypes.POINTER(CacheJob), c_int, ctypes.POINTER(BatchShape),
+                                             ctypes.POINTER(c_void_p)]
+        L.vbt_cache_batch_destroy.argtypes = [c_void_p]
+        L.vbt_cache_batch_launch.argtypes = [c_void_p, c_void_p]
+        L.vbt_cache_batch_launch_direct.argtypes = [c_void_p, c_void_p]
+        L.vbt_cache_batch_set_shape.argtypes = [c_void_p, ctypes.POINTER(BatchShape)]
+        L.vbt_cache_batch_get_shape.argtypes = [c_void_p, ctypes.POINTER(BatchShape)]
+        L.vbt_cache_batch_plan_info.argtypes = [c_void_p, ctypes.POINTER(c_u64), ctypes.POINTER(c_u32)]
+        L.vbt_cache_batch_autotune.argtypes = [c_void_p, c_int, ctypes.POINTER(BatchShape),
+                                               ctypes.POINTER(ctypes.c_float), c_void_p]
+        L.vbt_cache_batch_trial_count.argtypes = [c_void_p]
+        L.vbt_cache_batch_trial.argtypes = [c_void_p, c_int, ctypes.POINTER(BatchShape),
+                                            ctypes.POINTER(ctypes.c_float)]
         L.vbt_half_bits_to_float.restype = ctypes.c_float
         L.vbt_half_bits_to_float.argtypes = [ctypes.c_uint16]
         L.vbt_float_to_half_bits.restype = ctypes.c_uint16
@@ -328,8 +357,8 @@ class Asset:
         check(lib().vbt_asset_autotune(self.h, int(nt or self.dims[3]), reps, ms, ctypes.byref(best), self.stream.h))
         return best.value, [round(float(x), 4) for x in ms]
 
-    def frame_cache_into(self, out: DeviceArray, t0: int, nt: int, layout=CACHE_PACKED):
-        check(lib().vbt_decode_frame_cache(self.h, t0, nt, layout, out.ptr, self.stream.h))
+    def frame_cache_into(self, out: DeviceArray, t0: int, nt: int, layout=CACHE_PACKED, stream: Stream | None = None):
+        check(lib().vbt_decode_frame_cache(self.h, t0, nt, layout, out.ptr, (stream or self.stream).h))
 
     def frame_cache(self, t0: int, nt: int = 1, layout=CACHE_PACKED) -> np.ndarray:
         n = self.info.controlsPerFrame if layout == CACHE_PACKED else self.info.activeLeaves * 576
@@ -409,6 +438,69 @@ def decode_leaf_values(leaf_bytes: bytes, total_frames, idx, t, use_fine=False, 
 REQ_QUERY, REQ_CACHE = 0, 1
 
 
+def default_batch_shape() -> BatchShape:
+    s = BatchShape()
+    lib().vbt_batch_default_shape(ctypes.byref(s))
+    return s
+
+
+class CacheBatch:
+    """Frame-cache builds of several assets as one graph replay
+    (vbt_cache_batch_*).  jobs: (asset, t0, nt, layout, DeviceArray out)."""
+
+    def __init__(self, jobs, shape: BatchShape | None = None, stream: Stream | None = None):
+        self.stream = stream or Stream()
+        self.jobs = list(jobs)  # keeps assets and outputs alive
+        arr = (CacheJob * len(self.jobs))()
+        for i, (asset, t0, nt, layout, out) in enumerate(self.jobs):
+            arr[i] = CacheJob(asset.h, int(t0), int(nt), int(layout), out.ptr)
+        self.h = c_void_p()
+        check(lib().vbt_cache_batch_create(arr, len(self.jobs), ctypes.byref(shape) if shape else None,
+                                           ctypes.byref(self.h)))
+
+    def launch(self, graph=True):
+        fn = lib().vbt_cache_batch_launch if graph else lib().vbt_cache_batch_launch_direct
+        check(fn(self.h, self.stream.h))
+
+    @property
+    def shape(self) -> BatchShape:
+        s = BatchShape()
+        check(lib().vbt_cache_batch_get_shape(self.h, ctypes.byref(s)))
+        return s
+
+    @shape.setter
+    def shape(self, s: BatchShape):
+        check(lib().vbt_cache_batch_set_shape(self.h, ctypes.byref(s)))
+
+    def plan_info(self):
+        items, wg = c_u64(), c_u32()
+        check(lib().vbt_cache_batch_plan_info(self.h, ctypes.byref(items), ctypes.byref(wg)))
+        return items.value, wg.value
+
+    def autotune(self, reps: int = 3):
+        """Coordinate-descent over the queue shape; returns (best shape dict,
+        best ms, [(shape dict, ms), ...])."""
+        best, ms = BatchShape(), ctypes.c_float()
+        check(lib().vbt_cache_batch_autotune(self.h, reps, ctypes.byref(best), ctypes.byref(ms), self.stream.h))
+        trials = []
+        for i in range(lib().vbt_cache_batch_trial_count(self.h)):
+            s, t = BatchShape(), ctypes.c_float()
+            check(lib().vbt_cache_batch_trial(self.h, i, ctypes.byref(s), ctypes.byref(t)))
+            trials.append((s.as_dict(), round(float(t.value), 4)))
+        return best.as_dict(), float(ms.value), trials
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vbt_cache_batch_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """Continuous-batching decode server (vbt_server_* in the C-ABI)."""
 
@@ -484,3 +576,15 @@ class Server:
 def device_copy(dst: DeviceArray, src: DeviceArray, nbytes: int, stream: Stream | None = None):
     """Plain device copy kernel (the byte-counter reference for profiling)."""
     check(lib().vbt_device_copy(dst.ptr, src.ptr, nbytes, stream.h if stream else None))
+
+
+def device_fill(dst: DeviceArray, word: int, nbytes: int, stream: Stream | None = None):
+    check(lib().vbt_device_fill(dst.ptr, word & 0xFFFFFFFF, nbytes, stream.h if stream else None))
+
+
+def selftest_weights(max_gap: int = 65535, stream: Stream | None = None) -> int:
+    """Mismatching lerp weights of the frame-cache sweep over every gap up to
+    max_gap (0 expected)."""
+    n = c_u64()
+    check(lib().vbt_selftest_weights(max_gap, ctypes.byref(n), stream.h if stream else None))
+    return n.value
