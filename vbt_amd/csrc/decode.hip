@@ -352,7 +352,7 @@ template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
 __global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32_t* groupLeaf, int t0, int nt,
                                                         float* out, uint64_t total, uint32_t* errFlag)
 {
-    __shared__ uint32_t lds[WINDOW / 4];
+    __shared__ uint32_t lds[WINDOW / 4 + 1]; // + 1: see lds_key
     __shared__ uint32_t sRed[BLOCK / 64];
     cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed);
 }

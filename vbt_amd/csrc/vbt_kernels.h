@@ -378,20 +378,6 @@ __device__ __forceinline__ float sweep_half(uint32_t h)
     return ((h & 0x7C00u) == 0x7C00u && (h & 0x3FFu)) ? __uint_as_float(nanBits) : f;
 }
 
-template <uint32_t KB, bool NANFIX>
-__device__ __forceinline__ void sweep_key(const uint8_t* p, uint32_t i, int& f, float& v)
-{
-    if (KB == 8u) {
-        const uint8_t* q = p + i * 8u;
-        f = (int)((uint32_t)q[0] | ((uint32_t)q[1] << 8));
-        v = __uint_as_float((uint32_t)q[4] | ((uint32_t)q[5] << 8) | ((uint32_t)q[6] << 16) | ((uint32_t)q[7] << 24));
-    } else {
-        const uint8_t* q = p + i * 3u;
-        f = (int)q[0];
-        v = sweep_half<NANFIX>((uint32_t)q[1] | ((uint32_t)q[2] << 8));
-    }
-}
-
 // The forward sweep of one control (strictly increasing key frames).  State:
 // keys k, k+1 and k+2 (clamped to the last key) and 1 / (f[k+1] - f[k]).
 // Invariant after each frame: the next key's frame fn exceeds the frame just
@@ -421,41 +407,51 @@ __device__ __forceinline__ double sweep_recip(int gap)
     return __builtin_fma(r, e, r);
 }
 
-// State at the first requested frame from the control's records (g points at
-// key 0): the catch-up walk that establishes the invariant.
+typedef __attribute__((address_space(3))) const uint8_t lds_byte;
+
+// Record of a control staged in LDS at byte offset off (32-bit LDS addressing),
+// read with aligned dword loads: an unaligned 16-bit LDS read stalls the LDS
+// pipeline (the staged pool's 3-byte fp16 records sit at every byte offset),
+// so the record is cut out of the two dwords covering it.  The second dword
+// is only needed when the record crosses into it; the LDS array has one
+// padding dword after the window so the read stays inside it.  fp32 records
+// (8 bytes) are 4-byte aligned in the pool: frame in the low half of the
+// first dword, value in the second.
 template <uint32_t KB, bool NANFIX>
-__device__ __forceinline__ void sweep_start(const uint8_t* g, uint32_t last, int first, SweepState& s)
+__device__ __forceinline__ void lds_key(lds_byte* L, uint32_t off, int& f, float& v)
+{
+    typedef __attribute__((address_space(3))) const uint32_t lds_word;
+    lds_word* W = (lds_word*)L;
+    const uint32_t wi = off >> 2;
+    const uint32_t lo = W[wi], hi = W[wi + 1u];
+    if (KB == 8u) {
+        f = (int)(lo & 0xFFFFu);
+        v = __uint_as_float(hi);
+    } else {
+        const uint32_t x = __builtin_amdgcn_alignbyte(hi, lo, off & 3u); // bytes off .. off+3
+        f = (int)(x & 0xFFu);
+        v = sweep_half<NANFIX>((x >> 8) & 0xFFFFu);
+    }
+}
+
+// State at the first requested frame from the control's records staged in
+// LDS at byte offset off: the catch-up walk that establishes the invariant.
+template <uint32_t KB, bool NANFIX>
+__device__ __forceinline__ void sweep_start(lds_byte* L, uint32_t off, uint32_t last, int first, SweepState& s)
 {
     s.k = 0u;
     s.fn = 0;
     s.vn = 0.0f;
-    sweep_key<KB, NANFIX>(g, 0u, s.fk, s.vk);
-    if (last >= 1u) sweep_key<KB, NANFIX>(g, 1u, s.fn, s.vn);
+    lds_key<KB, NANFIX>(L, off, s.fk, s.vk);
+    if (last >= 1u) lds_key<KB, NANFIX>(L, off + KB, s.fn, s.vn);
     while (s.k < last && s.fn <= first) {
         ++s.k;
         s.fk = s.fn;
         s.vk = s.vn;
-        if (s.k < last) sweep_key<KB, NANFIX>(g, s.k + 1u, s.fn, s.vn);
+        if (s.k < last) lds_key<KB, NANFIX>(L, off + (s.k + 1u) * KB, s.fn, s.vn);
     }
     s.rinv = s.k < last ? sweep_recip(s.fn - s.fk) : 0.0;
-    sweep_key<KB, NANFIX>(g, s.k + 2u <= last ? s.k + 2u : last, s.fp, s.vp);
-}
-
-typedef __attribute__((address_space(3))) const uint8_t lds_byte;
-typedef uint16_t u16_any_align __attribute__((aligned(1)));
-
-// Record of a control staged in LDS at byte offset off (32-bit LDS addressing).
-template <uint32_t KB, bool NANFIX>
-__device__ __forceinline__ void lds_key(lds_byte* L, uint32_t off, int& f, float& v)
-{
-    if (KB == 8u) {
-        f = (int)((uint32_t)L[off] | ((uint32_t)L[off + 1u] << 8));
-        v = __uint_as_float((uint32_t)L[off + 4u] | ((uint32_t)L[off + 5u] << 8) | ((uint32_t)L[off + 6u] << 16) |
-                            ((uint32_t)L[off + 7u] << 24));
-    } else {
-        f = (int)L[off];
-        v = sweep_half<NANFIX>(*(__attribute__((address_space(3))) const u16_any_align*)(L + off + 1u));
-    }
+    lds_key<KB, NANFIX>(L, off + (s.k + 2u <= last ? s.k + 2u : last) * KB, s.fp, s.vp);
 }
 
 // Raw buffer descriptor of one frame of the output (base, size in bytes):
@@ -659,16 +655,15 @@ __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint3
     const uint64_t strideBytes = stride * 4u;
     auto run = [&](uint32_t off) {
         lds_byte* L = (lds_byte*)lds;
-        const uint8_t* rec = (const uint8_t*)lds + off;
         SweepState st;
         if (mode == 0u) {
-            sweep_start<8u, false>(rec, last, first, st);
+            sweep_start<8u, false>(L, off, last, first, st);
             sweep_frames<8u, false>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
         } else if (mode == 1u) {
-            sweep_start<3u, false>(rec, last, first, st);
+            sweep_start<3u, false>(L, off, last, first, st);
             sweep_frames<3u, false>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
         } else {
-            sweep_start<3u, true>(rec, last, first, st);
+            sweep_start<3u, true>(L, off, last, first, st);
             sweep_frames<3u, true>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
         }
     };
