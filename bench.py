@@ -157,7 +157,7 @@ def timed(hip, stream, fn, steps: int, warmup: int = 1) -> float:
     return e0.elapsed_ms(e1) / steps
 
 
-def batch_bench(hip, assets: dict, names, steps: int) -> dict:
+def batch_bench(hip, assets: dict, names, steps: int, tune_reps: int = 2) -> dict:
     """All frames of each named asset into its packed cache: one replay of
     the batch graph (persistent work-queue kernel, shape autotuned) against
     one plain launch per asset (each with its own autotuned tile)."""
@@ -180,7 +180,7 @@ def batch_bench(hip, assets: dict, names, steps: int) -> dict:
 
     plain_ms = timed(hip, s, plain, steps)
     batch = hip.CacheBatch(jobs, stream=s)
-    best, tune_ms, trials = batch.autotune(reps=2)
+    best, tune_ms, trials = batch.autotune(reps=tune_reps)
     items, wgs = batch.plan_info()
     graph_ms = timed(hip, s, batch.launch, steps)
     direct_ms = timed(hip, s, lambda: batch.launch(graph=False), steps)
@@ -268,6 +268,7 @@ def main():
     ap.add_argument("--no-queries", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-ceilings", action="store_true")
+    ap.add_argument("--batch-tune-reps", type=int, default=2)
     ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
                     help="assets of the batched frame-cache build (all frames of each, one graph replay)")
     ap.add_argument("--tile", type=int, default=-1, help="frame-cache tile shape index (default: autotune)")
@@ -379,7 +380,8 @@ def main():
         "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
-                     "bytes_per_launch": byt["total"], "kernel": "k_cache_render<0>",
+                     "bytes_per_launch": byt["total"],
+                     "kernel": f"k_cache_render<0, {shapes[best][0]}, {shapes[best][1]}>",
                      "device_ceilings_gbs": ceil},
         "queries": {"random_65536": qres, "random_256": small},
     }
@@ -392,9 +394,9 @@ def main():
             extra[n] = hip.Asset(synth.WORKLOADS[n].encode(threads=min(16, os.cpu_count() or 1)))
             print(f"[bench] asset {n}: encoded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     if not args.no_batch:
-        line["batch_single"] = batch_bench(hip, extra, [args.workload], args.steps)
+        line["batch_single"] = batch_bench(hip, extra, [args.workload], args.steps, args.batch_tune_reps)
         names = [x for x in args.batch_assets.split(",") if x]
-        line["batch"] = batch_bench(hip, extra, names, args.steps)
+        line["batch"] = batch_bench(hip, extra, names, args.steps, args.batch_tune_reps)
         bb = line["batch"]
         print(f"[bench] batch {names}: graph {bb['batched_graph']['ms']} ms vs plain "
               f"{bb['plain_per_asset']['ms']} ms ({bb['batched_graph']['decoded_gbs']} GB/s decoded), shape "

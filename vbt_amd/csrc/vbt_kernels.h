@@ -379,7 +379,8 @@ __device__ __forceinline__ float sweep_half(uint32_t h)
 }
 
 // The forward sweep of one control (strictly increasing key frames).  State:
-// keys k, k+1 and k+2 (clamped to the last key) and 1 / (f[k+1] - f[k]).
+// keys k and k+1 and 1 / (f[k+1] - f[k]); the loop also carries key k+2
+// (clamped to the last key).
 // Invariant after each frame: the next key's frame fn exceeds the frame just
 // produced.  Frames advance by at most one per step, so a single predicated
 // advance per step keeps the invariant (the key after the next is beyond fn
@@ -388,8 +389,8 @@ __device__ __forceinline__ float sweep_half(uint32_t h)
 // bracket the reference's hinted binary search returns for increasing keys.
 struct SweepState {
     uint32_t k;
-    int fk, fn, fp;
-    float vk, vn, vp;
+    int fk, fn;
+    float vk, vn;
     double rinv;
 };
 
@@ -417,21 +418,38 @@ typedef __attribute__((address_space(3))) const uint8_t lds_byte;
 // padding dword after the window so the read stays inside it.  fp32 records
 // (8 bytes) are 4-byte aligned in the pool: frame in the low half of the
 // first dword, value in the second.
-template <uint32_t KB, bool NANFIX>
-__device__ __forceinline__ void lds_key(lds_byte* L, uint32_t off, int& f, float& v)
+// The two dwords holding the record at byte offset off, and the record
+// decoded from them (split so a loop can issue the read well before it needs
+// the record).
+struct RawKey {
+    uint32_t lo, hi, shift;
+};
+
+__device__ __forceinline__ RawKey lds_fetch(lds_byte* L, uint32_t off)
 {
     typedef __attribute__((address_space(3))) const uint32_t lds_word;
     lds_word* W = (lds_word*)L;
     const uint32_t wi = off >> 2;
-    const uint32_t lo = W[wi], hi = W[wi + 1u];
+    return RawKey{W[wi], W[wi + 1u], off & 3u};
+}
+
+template <uint32_t KB, bool NANFIX>
+__device__ __forceinline__ void key_decode(const RawKey& r, int& f, float& v)
+{
     if (KB == 8u) {
-        f = (int)(lo & 0xFFFFu);
-        v = __uint_as_float(hi);
+        f = (int)(r.lo & 0xFFFFu);
+        v = __uint_as_float(r.hi);
     } else {
-        const uint32_t x = __builtin_amdgcn_alignbyte(hi, lo, off & 3u); // bytes off .. off+3
+        const uint32_t x = __builtin_amdgcn_alignbyte(r.hi, r.lo, r.shift); // bytes off .. off+3
         f = (int)(x & 0xFFu);
         v = sweep_half<NANFIX>((x >> 8) & 0xFFFFu);
     }
+}
+
+template <uint32_t KB, bool NANFIX>
+__device__ __forceinline__ void lds_key(lds_byte* L, uint32_t off, int& f, float& v)
+{
+    key_decode<KB, NANFIX>(lds_fetch(L, off), f, v);
 }
 
 // State at the first requested frame from the control's records staged in
@@ -451,7 +469,6 @@ __device__ __forceinline__ void sweep_start(lds_byte* L, uint32_t off, uint32_t 
         if (s.k < last) lds_key<KB, NANFIX>(L, off + (s.k + 1u) * KB, s.fn, s.vn);
     }
     s.rinv = s.k < last ? sweep_recip(s.fn - s.fk) : 0.0;
-    lds_key<KB, NANFIX>(L, off + (s.k + 2u <= last ? s.k + 2u : last) * KB, s.fp, s.vp);
 }
 
 // Raw buffer descriptor of one frame of the output (base, size in bytes):
@@ -479,18 +496,21 @@ __device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t
     uint32_t k = s.k;
     int fk = s.fk, fn = s.fn;
     float vk = s.vk, vn = s.vn;
-    int fp = s.fp;
-    float vp = s.vp;
     double rinv = s.rinv;
     const uint32_t lastOff = off + last * KB;
-    // byte offset of record min(k + 2, last); the record itself is (fp, vp)
+    // byte offset of record min(k + 2, last); the record is carried raw (the
+    // LDS read in flight) and decoded where the next step needs it
     uint32_t pOff = min(off + (k + 2u) * KB, lastOff);
+    RawKey raw = lds_fetch(L, pOff);
     // step i's values for frame i's output, produced in step i + 1
     int fkS = 0;
     float vkS = 0.0f, vnS = 0.0f;
     bool endS = false;
     double rinvS = 0.0;
     auto update = [&](int frame) {
+        int fp;
+        float vp;
+        key_decode<KB, NANFIX>(raw, fp, vp);
         const bool adv = (k < last) & (fn <= frame);
         const uint32_t step = adv ? 1u : 0u;
         const double rNew = sweep_recip(fp - fn);
@@ -501,7 +521,7 @@ __device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t
         k += step;
         rinv = adv ? rNew : rinv;
         pOff = min(pOff + step * KB, lastOff);
-        lds_key<KB, NANFIX>(L, pOff, fp, vp);
+        raw = lds_fetch(L, pOff);
         fkS = fk;
         vkS = vk;
         vnS = vn;
