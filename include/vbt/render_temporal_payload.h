@@ -1,6 +1,8 @@
 // Drop-in for src/render_temporal_payload.h (the reference's leaf format API):
-// same namespace, type names, field order and function signatures, backed by
-// libvbt_hip.so (link with -lvbt_hip).  Format errors throw
+// same namespace, type names, field order and function signatures.  The
+// functions are defined in libvbt_hip_cxx.so over libvbt_hip.so's C-ABI (link
+// -lvbt_hip_cxx -lvbt_hip); validateRenderTemporalLeafPrefix, which reads the
+// encoder's option struct, is defined inline here.  Format errors throw
 // std::runtime_error with the reference's message text.
 #pragma once
 
@@ -147,32 +149,15 @@ inline std::vector<uint8_t> sized(int rc, std::vector<uint8_t>& buf, size_t len)
 }
 } // namespace hip_detail
 
-inline uint32_t renderTemporalKeyframeBytes(RenderTemporalSequenceCodec codec)
-{
-    return vbt_keyframe_bytes(static_cast<uint32_t>(codec));
-}
+uint32_t renderTemporalKeyframeBytes(RenderTemporalSequenceCodec codec);
 
-inline uint32_t packRenderTemporalHeader(const RenderTemporalPackedHeaderFields& fields)
-{
-    const vbt_header_fields h = hip_detail::to_c(fields);
-    return vbt_pack_header(&h);
-}
+uint32_t packRenderTemporalHeader(const RenderTemporalPackedHeaderFields& fields);
 
-inline RenderTemporalPackedHeaderFields unpackRenderTemporalHeader(uint32_t packedHeader)
-{
-    vbt_header_fields h;
-    vbt_unpack_header(packedHeader, &h);
-    return hip_detail::from_c(h);
-}
+RenderTemporalPackedHeaderFields unpackRenderTemporalHeader(uint32_t packedHeader);
 
-inline RenderTemporalPackedLeafPrefix makeRenderTemporalLeafPrefix(TemporalFirstPackedMode mode,
+RenderTemporalPackedLeafPrefix makeRenderTemporalLeafPrefix(TemporalFirstPackedMode mode,
                                                                    uint32_t coarseKeyframeCount,
-                                                                   uint32_t fineKeyframeCount)
-{
-    vbt_leaf_prefix p;
-    hip_detail::check(vbt_make_leaf_prefix(static_cast<uint32_t>(mode), coarseKeyframeCount, fineKeyframeCount, &p));
-    return RenderTemporalPackedLeafPrefix{p.coarseKeyframeCount, p.fineKeyframeCount};
-}
+                                                                   uint32_t fineKeyframeCount);
 
 inline bool validateRenderTemporalLeafPrefix(const RenderTemporalPackedLeafPrefix& prefix,
                                              const RenderTemporalPackedHeaderFields& header,
@@ -183,105 +168,31 @@ inline bool validateRenderTemporalLeafPrefix(const RenderTemporalPackedLeafPrefi
     return vbt_validate_leaf_prefix(&p, &h, options.timeBinCount) != 0;
 }
 
-inline RenderTemporalPackedLeafLayout computeRenderTemporalLeafLayout(const RenderTemporalPackedHeaderFields& header,
+RenderTemporalPackedLeafLayout computeRenderTemporalLeafLayout(const RenderTemporalPackedHeaderFields& header,
                                                                      const RenderTemporalPackedLeafPrefix& prefix,
-                                                                     bool alignTo4Bytes)
-{
-    const vbt_leaf_prefix p = hip_detail::to_c(prefix);
-    const vbt_header_fields h = hip_detail::to_c(header);
-    vbt_leaf_layout l;
-    vbt_compute_leaf_layout(&h, &p, alignTo4Bytes ? 1 : 0, &l);
-    return hip_detail::from_c(l);
-}
+                                                                     bool alignTo4Bytes);
 
-inline std::vector<uint8_t> buildRenderTemporalPackedLeafBytes(const RenderTemporalPackedHeaderFields& header,
+std::vector<uint8_t> buildRenderTemporalPackedLeafBytes(const RenderTemporalPackedHeaderFields& header,
                                                                const RenderTemporalPackedLeafPrefix& prefix,
                                                                const std::vector<uint8_t>& coarseStreamBytes,
                                                                const std::vector<uint8_t>& fineStreamBytes,
-                                                               bool alignTo4Bytes)
-{
-    const vbt_leaf_prefix p = hip_detail::to_c(prefix);
-    const vbt_header_fields h = hip_detail::to_c(header);
-    size_t len = 0;
-    hip_detail::check(vbt_build_leaf_bytes(&h, &p, coarseStreamBytes.data(), coarseStreamBytes.size(),
-                                           fineStreamBytes.data(), fineStreamBytes.size(), alignTo4Bytes ? 1 : 0,
-                                           nullptr, 0, &len));
-    std::vector<uint8_t> out(len);
-    return hip_detail::sized(vbt_build_leaf_bytes(&h, &p, coarseStreamBytes.data(), coarseStreamBytes.size(),
-                                                  fineStreamBytes.data(), fineStreamBytes.size(),
-                                                  alignTo4Bytes ? 1 : 0, out.data(), out.size(), &len),
-                             out, len);
-}
+                                                               bool alignTo4Bytes);
 
-inline std::vector<uint8_t> buildRenderTemporalPackedLeafBytes(const RenderTemporalPackedHeaderFields& header,
+std::vector<uint8_t> buildRenderTemporalPackedLeafBytes(const RenderTemporalPackedHeaderFields& header,
                                                                const RenderTemporalPackedLeafPrefix& prefix,
                                                                const std::vector<uint8_t>& coarseStreamBytes,
                                                                const std::vector<uint8_t>& shellOccupancyBytes,
                                                                const std::vector<uint8_t>& shellResidualStreamBytes,
-                                                               bool alignTo4Bytes)
-{
-    const vbt_leaf_prefix p = hip_detail::to_c(prefix);
-    const vbt_header_fields h = hip_detail::to_c(header);
-    size_t len = 0;
-    auto call = [&](uint8_t* out, size_t cap) {
-        return vbt_build_shell_leaf_bytes(&h, &p, coarseStreamBytes.data(), coarseStreamBytes.size(),
-                                          shellOccupancyBytes.data(), shellOccupancyBytes.size(),
-                                          shellResidualStreamBytes.data(), shellResidualStreamBytes.size(),
-                                          alignTo4Bytes ? 1 : 0, out, cap, &len);
-    };
-    hip_detail::check(call(nullptr, 0));
-    std::vector<uint8_t> out(len);
-    return hip_detail::sized(call(out.data(), out.size()), out, len);
-}
+                                                               bool alignTo4Bytes);
 
-inline std::vector<uint8_t> packRenderTemporalShellOccupancySection(const RenderTemporalShellOccupancySection& section)
-{
-    vbt_shell_section s;
-    for (int i = 0; i < 8; ++i) {
-        s.shellMask[i] = section.shellMask[i];
-        s.groupPrefix[i] = section.groupPrefix[i];
-    }
-    std::vector<uint8_t> out(80);
-    vbt_pack_shell_section(&s, out.data());
-    return out;
-}
+std::vector<uint8_t> packRenderTemporalShellOccupancySection(const RenderTemporalShellOccupancySection& section);
 
-inline RenderTemporalShellOccupancySection unpackRenderTemporalShellOccupancySection(const uint8_t* data,
-                                                                                     std::uint64_t size)
-{
-    vbt_shell_section s;
-    vbt_unpack_shell_section(data, size, &s);
-    RenderTemporalShellOccupancySection o;
-    for (int i = 0; i < 8; ++i) {
-        o.shellMask[i] = s.shellMask[i];
-        o.groupPrefix[i] = s.groupPrefix[i];
-    }
-    return o;
-}
+RenderTemporalShellOccupancySection unpackRenderTemporalShellOccupancySection(const uint8_t* data,
+                                                                                     std::uint64_t size);
 
-inline std::uint16_t renderTemporalShellActiveVoxelCount(const RenderTemporalShellOccupancySection& section)
-{
-    vbt_shell_section s;
-    for (int i = 0; i < 8; ++i) {
-        s.shellMask[i] = section.shellMask[i];
-        s.groupPrefix[i] = section.groupPrefix[i];
-    }
-    return vbt_shell_active_count(&s);
-}
+std::uint16_t renderTemporalShellActiveVoxelCount(const RenderTemporalShellOccupancySection& section);
 
-inline RenderTemporalPackedLeafView parseRenderTemporalPackedLeaf(const uint8_t* leafBase, size_t leafBytes,
-                                                                  bool alignTo4Bytes)
-{
-    vbt_leaf_view v;
-    hip_detail::check(vbt_parse_leaf(leafBase, leafBytes, alignTo4Bytes ? 1 : 0, &v));
-    RenderTemporalPackedLeafView o;
-    o.header = hip_detail::from_c(v.header);
-    o.prefix = RenderTemporalPackedLeafPrefix{v.prefix.coarseKeyframeCount, v.prefix.fineKeyframeCount};
-    o.layout = hip_detail::from_c(v.layout);
-    o.shellActiveVoxelCount = v.shellActiveVoxelCount;
-    o.shellResidualOffset = v.shellResidualOffset;
-    o.shellResidualBytes = v.shellResidualBytes;
-    return o;
-}
+RenderTemporalPackedLeafView parseRenderTemporalPackedLeaf(const uint8_t* leafBase, size_t leafBytes,
+                                                                  bool alignTo4Bytes);
 
 } // namespace vbt

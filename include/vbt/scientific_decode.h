@@ -1,6 +1,6 @@
 // Drop-in for render/src/scientific_decode.h and the Query4D record of
 // render/src/query_patterns.h:11-17, with the reference's names and
-// signatures, decoding on the GPU through libvbt_hip.so.
+// signatures, decoding on the GPU (defined in libvbt_hip_cxx.so).
 //
 // Each call uploads the file to HBM, decodes, and frees it again: correct
 // for a drop-in, but callers that decode more than once keep a
@@ -14,8 +14,17 @@
 #include <stdexcept>
 #include <vector>
 
+// Inside the reference's tree, query_patterns.h (render/src) is on the include
+// path and declares Query4D and the query generators (its .cpp stays in the
+// build: it is not decode code).  Standalone, the record is defined here.
+#if __has_include("query_patterns.h")
+#include "query_patterns.h"
+#define VBT_HIP_HAVE_QUERY_PATTERNS 1
+#endif
+
 namespace vbt::render {
 
+#ifndef VBT_HIP_HAVE_QUERY_PATTERNS
 struct Query4D {  // render/src/query_patterns.h:11-17
     uint32_t x = 0;
     uint32_t y = 0;
@@ -29,32 +38,14 @@ inline void fillLeafIndices(const VbtFileHeader& header, std::vector<Query4D>& q
 {
     for (auto& q : queries) q.leafIndex = leafIndexForVoxel(header, q.x, q.y, q.z);
 }
+#endif
 
 // decodeScientificValueAtCpu (render/src/scientific_decode.h:11).  A query
 // outside the volume or at t >= frames throws.
-inline float decodeScientificValueAtCpu(const VbtFile& file, const Query4D& query)
-{
-    vbt::hip::Asset asset(file);
-    const std::vector<uint32_t> q = {query.x, query.y, query.z, query.t};
-    vbt::hip::DeviceBuffer<uint32_t> dq(4);
-    vbt::hip::DeviceBuffer<float> out(1);
-    vbt::hip::DeviceBuffer<int32_t> st(1);
-    dq.upload(q.data(), 4);
-    asset.query(dq.data(), 1, out.data(), st.data());
-    const int32_t status = st.download()[0];
-    if (status != 0) throw std::runtime_error(vbt_status_message(status));
-    return out.download()[0];
-}
+float decodeScientificValueAtCpu(const VbtFile& file, const Query4D& query);
 
 // reconstructScientificFrameCpu (render/src/scientific_decode.h:12): frame
 // t as row-major (z, y, x) floats.
-inline std::vector<float> reconstructScientificFrameCpu(const VbtFile& file, uint32_t frameIndex)
-{
-    vbt::hip::Asset asset(file);
-    const auto& h = asset.info().header;
-    vbt::hip::DeviceBuffer<float> out(static_cast<size_t>(h.width) * h.height * h.depth);
-    asset.denseFrame(static_cast<int>(frameIndex), out.data());
-    return out.download();
-}
+std::vector<float> reconstructScientificFrameCpu(const VbtFile& file, uint32_t frameIndex);
 
 } // namespace vbt::render

@@ -1,8 +1,9 @@
 // Drop-in for render/src/vbt_file.h: the VBTPACK4 container types and loader
-// with the reference's names and signatures (namespace vbt::render), backed
-// by libvbt_hip.so.  loadVbtFile runs the library's container validation
-// (the reference's checks and error texts) and keeps the file's arrays on the
-// host, as the reference does; vbt::hip::Asset (hip_asset.h) puts one in HBM.
+// with the reference's names and signatures (namespace vbt::render), defined
+// in libvbt_hip_cxx.so over libvbt_hip.so.  loadVbtFile runs the library's
+// container validation (the reference's checks and error texts) and keeps the
+// file's arrays on the host, as the reference does; vbt::hip::Asset
+// (hip_asset.h) puts one in HBM.
 #pragma once
 
 #include "../vbt_hip.h"
@@ -92,92 +93,13 @@ inline std::vector<uint8_t> image(const VbtFile& f)
 } // namespace hip_detail
 
 // loadVbtFile (render/src/vbt_file.cpp:109-210): false + error text on failure.
-inline bool loadVbtFile(const std::filesystem::path& path, VbtFile& outFile, std::string& error)
-{
-    error.clear();
-    std::error_code ec;
-    const uintmax_t size = std::filesystem::file_size(path, ec);
-    if (ec) {
-        error = "Failed to query file size: " + path.string();
-        return false;
-    }
-    if (size < 52u) {
-        error = "Invalid VBTPACK4 file size.";
-        return false;
-    }
-    std::ifstream in(path, std::ios::binary);
-    if (!in) {
-        error = "Failed to open file: " + path.string();
-        return false;
-    }
-    std::vector<uint8_t> bytes(static_cast<size_t>(size));
-    in.read(reinterpret_cast<char*>(bytes.data()), static_cast<std::streamsize>(bytes.size()));
-    if (!in) {
-        error = "Failed to read VBTPACK header.";
-        return false;
-    }
-    vbt_file_header h;
-    if (vbt_validate_file(bytes.data(), bytes.size(), &h) != VBT_OK) {
-        error = vbt_last_error();
-        return false;
-    }
-    VbtFile f;
-    std::memcpy(f.header.magic, bytes.data(), 8);
-    f.header.version = h.version;
-    f.header.width = h.width;
-    f.header.height = h.height;
-    f.header.depth = h.depth;
-    f.header.frames = h.frames;
-    f.header.leafSize = h.leafSize;
-    f.header.coarseResolution = h.coarseResolution;
-    f.header.maxCoarseKeep = h.maxCoarseKeep;
-    f.header.leafCount = h.leafCount;
-    f.header.profileType = h.profileType;
-    f.header.coarseAcScaleCount = h.coarseAcScaleCount;
-    size_t o = 52;
-    f.coarseAcScales.resize(h.coarseAcScaleCount);
-    std::memcpy(f.coarseAcScales.data(), bytes.data() + o, 4u * h.coarseAcScaleCount);
-    o += 4u * h.coarseAcScaleCount;
-    f.offsetsWords.resize(static_cast<size_t>(h.leafCount) + 1u);
-    std::memcpy(f.offsetsWords.data(), bytes.data() + o, 4u * f.offsetsWords.size());
-    o += 4u * f.offsetsWords.size();
-    f.payloadWords.resize((bytes.size() - o) / 4u);
-    std::memcpy(f.payloadWords.data(), bytes.data() + o, 4u * f.payloadWords.size());
-    outFile = std::move(f);
-    return true;
-}
+bool loadVbtFile(const std::filesystem::path& path, VbtFile& outFile, std::string& error);
 
 // readPackedHeaderWord (render/src/vbt_file.cpp:212-218)
-inline uint32_t readPackedHeaderWord(const VbtFile& file, uint32_t leafIndex)
-{
-    if (leafIndex >= file.header.leafCount) return 0u;
-    const uint32_t w = file.offsetsWords[leafIndex];
-    return w < file.payloadWords.size() ? file.payloadWords[w] : 0u;
-}
+uint32_t readPackedHeaderWord(const VbtFile& file, uint32_t leafIndex);
 
-inline ScientificHeaderV2 decodeScientificHeaderV2(uint32_t packedHeader)
-{
-    vbt_scientific_header s;
-    vbt_decode_scientific_header(packedHeader, &s);
-    ScientificHeaderV2 o;
-    o.mode = static_cast<ScientificMode>(s.mode);
-    o.coarseKeepMinus1 = s.coarseKeepMinus1;
-    o.coarseQuantClass = s.coarseQuantClass;
-    o.tierId = s.tierId;
-    o.framesPerBinMinus1 = s.framesPerBinMinus1;
-    o.eventQuantCode = s.eventQuantCode;
-    o.sparseSubtype = s.sparseSubtype;
-    o.packedEventCount = s.packedEventCount;
-    o.denseQuantBits = s.denseQuantBits;
-    o.denseSubtype = s.denseSubtype;
-    o.denseTemporalKeepMinus1 = s.denseTemporalKeepMinus1;
-    return o;
-}
+ScientificHeaderV2 decodeScientificHeaderV2(uint32_t packedHeader);
 
-inline uint32_t leafIndexForVoxel(const VbtFileHeader& header, uint32_t x, uint32_t y, uint32_t z)
-{
-    const vbt_file_header h = hip_detail::to_c(header);
-    return vbt_leaf_index_for_voxel(&h, x, y, z);
-}
+uint32_t leafIndexForVoxel(const VbtFileHeader& header, uint32_t x, uint32_t y, uint32_t z);
 
 } // namespace vbt::render

@@ -4,6 +4,7 @@
 
   vbt_amd/libvbt_hip.so    HIP decode library (gfx950), C-ABI in include/vbt_hip.h
   vbt_amd/libvbt_synth.so  synthetic fields + asset encoder (host C++, OpenMP)
+  vbt_amd/libvbt_hip_cxx.so  the reference's C++ decode API over the C-ABI (host C++)
   vbt_amd/libvbt_loadgen.so  open-loop load generator over libvbt_hip.so (host C++)
   tests/native/shim_check  consumer of the C++ drop-in headers (test program)
   oracle/liboracle.so      CPU oracle (test infrastructure)
@@ -98,14 +99,27 @@ def build_loadgen():
     return out
 
 
+def build_cxx():
+    """vbt_amd/libvbt_hip_cxx.so: the reference's C++ decode API as exported
+    functions over the C-ABI (links libvbt_hip.so)."""
+    out = os.path.join(HERE, "libvbt_hip_cxx.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-shared", "-D__HIP_PLATFORM_AMD__",
+          f"-I{ROCM}/include", "-o", tmp, os.path.join(CSRC, "vbt_cxx.cpp"), f"-L{HERE}", "-lvbt_hip",
+          f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_shim_check():
     """tests/native/shim_check: a consumer of the C++ drop-in headers
-    (include/vbt/*.h), used by tests/test_cpp_shim.py."""
+    (include/vbt/*.h) linked against libvbt_hip_cxx.so, used by
+    tests/test_cpp_shim.py."""
     out = os.path.join(ROOT, "tests", "native", "shim_check")
     tmp = out + ".tmp"
     _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-o", tmp,
-          os.path.join(ROOT, "tests", "native", "shim_check.cpp"), f"-L{HERE}", "-lvbt_hip", f"-L{ROCM}/lib",
-          "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../vbt_amd", f"-Wl,-rpath,{ROCM}/lib"])
+          os.path.join(ROOT, "tests", "native", "shim_check.cpp"), f"-L{HERE}", "-lvbt_hip_cxx", "-lvbt_hip",
+          f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../vbt_amd", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, out)
     return out
 
@@ -117,16 +131,26 @@ def build_oracle():
         _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "-j4"])
 
 
+def build_dropin():
+    """The reference's own unit tests and a render tool linked against
+    libvbt_hip_cxx.so (oracle/_ref/ref_*; needs the reference sources)."""
+    if os.path.isdir("/root/reference"):
+        _run(["make", "-C", os.path.join(ROOT, "oracle"), "dropin", "-j4"])
+
+
 def build_all():
     build_synth()
     build_oracle()
     build_hip()
+    build_cxx()
     build_loadgen()
     build_shim_check()
+    build_dropin()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "hip": build_hip, "synth": build_synth, "oracle": build_oracle,
-         "loadgen": build_loadgen, "shim": build_shim_check}[w]()
+         "loadgen": build_loadgen, "cxx": build_cxx, "shim": build_shim_check,
+         "dropin": build_dropin}[w]()
