@@ -374,6 +374,8 @@ typedef struct {
     uint32_t cacheItem;      /* packed-cache slots per work item, multiple of 256 (default 1024) */
     uint32_t idleUs;         /* device loop exits after this long without new requests (default 2000) */
     uint32_t leaseMs;        /* device loop exits (and is relaunched) at least this often (default 250) */
+    uint32_t frameBuildsInLoop; /* 0 (default): whole-frame builds run as one-shot frame-cache kernels on a
+                                   stream beside the device loop; 1: as cacheItem-slot work items of the loop */
 } vbt_server_config;
 
 typedef struct {
@@ -383,7 +385,7 @@ typedef struct {
     const float* d_values;   /* device pointer, valid until fetch/release */
     const int32_t* d_status; /* per-query statuses (queries) or NULL */
     double latencyUs;        /* submit -> completion observed by the dispatcher */
-    uint64_t launch;         /* device-loop launch it ran in */
+    uint64_t launch;         /* device-loop launch it ran in (0: a frame build beside the loop) */
 } vbt_request_result;
 
 typedef struct {

@@ -64,7 +64,8 @@ def submit(srv, ids, req):
     return srv.submit_query(ids[name], arg) if kind == "q" else srv.submit_cache(ids[name], arg)
 
 
-def test_alone_equals_mix(assets):
+@pytest.mark.parametrize("in_loop", [0, 1], ids=["frames-beside-loop", "frames-in-loop"])
+def test_alone_equals_mix(assets, in_loop):
     rng = np.random.default_rng(2024)
     probes = []
     for n in NAMES:
@@ -82,7 +83,7 @@ def test_alone_equals_mix(assets):
             mix.append(("q", n, rand_queries(rng, dims, max(1, size))))
     ref = {i: expected(assets, r) for i, r in enumerate(probes + mix)}
 
-    srv = hip.Server()
+    srv = hip.Server(frameBuildsInLoop=in_loop)
     ids = {n: srv.register(assets[n][0]) for n in NAMES}
     # probes alone, one at a time
     alone = []
@@ -297,7 +298,7 @@ def test_largest_query_batch_and_item_shapes(assets):
     exp = a.query(q)
     same(exp[:5000], o.query(q[:5000]), "one-shot vs oracle")
     for qi, ci, bpc in ((32, 256, 1), (128, 1024, 2), (512, 4096, 4)):
-        srv = hip.Server(queryItem=qi, cacheItem=ci, blocksPerCU=bpc)
+        srv = hip.Server(queryItem=qi, cacheItem=ci, blocksPerCU=bpc, frameBuildsInLoop=1)
         i = srv.register(a)
         same(srv.fetch(srv.submit_query(i, q)), exp, f"65536 queries, item {qi}")
         same(srv.fetch(srv.submit_cache(i, 9)), a.frame_cache(9, 1)[0], f"cache, item {ci}")

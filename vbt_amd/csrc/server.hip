@@ -535,6 +535,14 @@ struct vbt_server_s {
     std::vector<std::shared_ptr<Req>> inflight;  // admitted, not complete
     bool loopRunning = false;
     uint32_t ringTail = 0;
+    // whole-frame builds launched as one-shot kernels beside the loop
+    hipStream_t frameStream = nullptr;
+    std::vector<hipEvent_t> eventPool;
+    struct FrameBuild {
+        std::shared_ptr<Req> r;
+        hipEvent_t done;
+    };
+    std::vector<FrameBuild> frames;
     uint64_t launchSeq = 0;
     uint64_t nextId = 1;
     bool stopping = false, pauseLoop = false, paused = false;
@@ -552,7 +560,51 @@ struct vbt_server_s {
     bool launch_loop();
     void end_of_launch();
     void admit();
+    bool launch_frame(const std::shared_ptr<Req>& r);
+    void poll_frames(Clock::time_point now);
 };
+
+// A whole-frame build as the one-shot frame-cache kernel on the frame stream
+// (it runs beside the device loop on the CUs the loop leaves free); its event
+// marks completion.  False when the launch itself failed.
+bool vbt_server_s::launch_frame(const std::shared_ptr<Req>& r)
+{
+    hipEvent_t ev = nullptr;
+    if (!eventPool.empty()) {
+        ev = eventPool.back();
+        eventPool.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        return false;
+    }
+    const int rc = vbt_decode_frame_cache(assets[r->asset], r->t, 1, VBT_CACHE_PACKED,
+                                          reinterpret_cast<float*>(dArena + r->devOff), frameStream);
+    // VBT_ERR_FORMAT only reports poisoned leaves after launching (complete()
+    // turns it into the request's status)
+    if ((rc != VBT_OK && rc != VBT_ERR_FORMAT) || hipEventRecord(ev, frameStream) != hipSuccess) {
+        eventPool.push_back(ev);
+        return false;
+    }
+    frames.push_back(FrameBuild{r, ev});
+    return true;
+}
+
+void vbt_server_s::poll_frames(Clock::time_point now)
+{
+    size_t w = 0;
+    bool any = false;
+    for (size_t i = 0; i < frames.size(); ++i) {
+        const hipError_t e = hipEventQuery(frames[i].done);
+        if (e == hipErrorNotReady) {
+            frames[w++] = frames[i];
+            continue;
+        }
+        complete(frames[i].r, e == hipSuccess ? VBT_OK : VBT_ERR_DEVICE, now);
+        eventPool.push_back(frames[i].done);
+        any = true;
+    }
+    frames.resize(w);
+    if (any) doneCv.notify_all();
+}
 
 void vbt_server_s::complete(const std::shared_ptr<Req>& r, int32_t status, Clock::time_point now)
 {
@@ -625,7 +677,7 @@ void vbt_server_s::end_of_launch()
 void vbt_server_s::admit()
 {
     const uint32_t first = ringTail;
-    while (!queue.empty() && ringTail < RING && inflight.size() < cfg.maxInflight) {
+    while (!queue.empty() && ringTail < RING && inflight.size() + frames.size() < cfg.maxInflight) {
         auto r = queue.front();
         const uint64_t need = out_bytes(*r);
         if (!r->hasDev && need) {
@@ -638,6 +690,15 @@ void vbt_server_s::admit()
         if (r->items == 0) {
             complete(r, VBT_OK, Clock::now());
             doneCv.notify_all();
+            continue;
+        }
+        if (r->type == REQ_CACHE && !cfg.frameBuildsInLoop) {
+            r->state = 1;
+            r->launch = 0;
+            if (!launch_frame(r)) {
+                complete(r, VBT_ERR_DEVICE, Clock::now());
+                doneCv.notify_all();
+            }
             continue;
         }
         RingEntry& e = ring[ringTail];
@@ -657,7 +718,7 @@ void vbt_server_s::admit()
         r->launch = loopRunning ? launchSeq : launchSeq + 1u;
         inflight.push_back(r);
     }
-    maxInflightSeen = std::max<uint64_t>(maxInflightSeen, inflight.size());
+    maxInflightSeen = std::max<uint64_t>(maxInflightSeen, inflight.size() + frames.size());
     if (ringTail == first) return;
     if (!loopRunning) {
         __atomic_store_n(&hs->stop, 0u, __ATOMIC_RELAXED);
@@ -682,6 +743,7 @@ void vbt_server_s::run()
     for (;;) {
         // a finished launch
         if (loopRunning && hipEventQuery(loopDone) == hipSuccess) end_of_launch();
+        if (!frames.empty()) poll_frames(Clock::now());
         // completions
         if (!inflight.empty()) {
             const auto now = Clock::now();
@@ -698,7 +760,7 @@ void vbt_server_s::run()
             }
         }
         if (stopping) {
-            if (!loopRunning) break;
+            if (!loopRunning && frames.empty()) break;
             __atomic_store_n(&hs->stop, 1u, __ATOMIC_RELEASE);
         } else if (pauseLoop) {
             // asset registration: end the loop, then wait until resumed
@@ -714,7 +776,7 @@ void vbt_server_s::run()
         } else if (fatal.empty() && !(loopRunning && __atomic_load_n(&hs->exited, __ATOMIC_ACQUIRE))) {
             admit();
         }
-        if (!loopRunning && (queue.empty() || !fatal.empty()) && !stopping && !pauseLoop) {
+        if (!loopRunning && frames.empty() && (queue.empty() || !fatal.empty()) && !stopping && !pauseLoop) {
             cv.wait(lk);
         } else {
             lk.unlock();
@@ -738,6 +800,7 @@ void vbt_server_default_config(vbt_server_config* c)
     c->cacheItem = 1024;
     c->idleUs = 2000;
     c->leaseMs = 250;
+    c->frameBuildsInLoop = 0;
 }
 
 int vbt_server_destroy(vbt_server s);
@@ -782,6 +845,7 @@ int vbt_server_create(const vbt_server_config* c, vbt_server* out)
 
     SCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     SCHK(hipStreamCreateWithFlags(&s->fetchStream, hipStreamNonBlocking));
+    SCHK(hipStreamCreateWithFlags(&s->frameStream, hipStreamNonBlocking));
     SCHK(hipEventCreateWithFlags(&s->loopDone, hipEventDisableTiming));
     SCHK(hipMalloc((void**)&s->dAssets, sizeof(AssetDev) * MAX_ASSETS));
     SCHK(hipMalloc((void**)&s->dArena, cfg.arenaBytes));
@@ -818,6 +882,7 @@ int vbt_server_destroy(vbt_server s)
     if (s->dispatcher.joinable()) s->dispatcher.join();
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->fetchStream) (void)hipStreamSynchronize(s->fetchStream);
+    if (s->frameStream) (void)hipStreamSynchronize(s->frameStream);
     {
         std::lock_guard<std::mutex> g(s->mu);
         for (auto& kv : s->live)
@@ -839,6 +904,9 @@ int vbt_server_destroy(vbt_server s)
     if (s->dLists) (void)hipFree(s->dLists);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->fetchStream) (void)hipStreamDestroy(s->fetchStream);
+    if (s->frameStream) (void)hipStreamDestroy(s->frameStream);
+    for (auto& f : s->frames) (void)hipEventDestroy(f.done);
+    for (hipEvent_t e : s->eventPool) (void)hipEventDestroy(e);
     delete s;
     return VBT_OK;
 }
