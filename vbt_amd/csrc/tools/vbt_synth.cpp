@@ -538,9 +538,29 @@ float series_eps(const float* s, int F, const vbts_encode_opts& o)
     return std::max(o.epsAbs, o.epsRel * (mx - mn));
 }
 
-void build_stream(const std::vector<float>& series, int C, int F, int B, int codec, float epsScale,
-                  const vbts_encode_opts& o, uint64_t seed, Stream& st)
+// The series a control's keyframes decode to: the first key before it, the
+// last after it, v0 + a (v1 - v0) between (fp16 keys rounded as stored).
+void reconstruct(const float* s, const std::vector<int>& keys, int F, int codec, float* rec)
 {
+    auto val = [&](int k) { return codec == 1 ? s[k] : from_half(to_half(s[k])); };
+    size_t i = 0;
+    for (int t = 0; t < F; ++t) {
+        while (i + 1 < keys.size() && keys[i + 1] <= t) ++i;
+        if (t <= keys[i] || i + 1 >= keys.size()) {
+            rec[t] = val(keys[i]);
+            continue;
+        }
+        const int f0 = keys[i], f1 = keys[i + 1];
+        const float a = (float)(t - f0) / (float)(f1 - f0);
+        const float v0 = val(f0), v1 = val(f1);
+        rec[t] = v0 + a * (v1 - v0);
+    }
+}
+
+void build_stream(const std::vector<float>& series, int C, int F, int B, int codec, float epsScale,
+                  const vbts_encode_opts& o, uint64_t seed, Stream& st, std::vector<float>* rec = nullptr)
+{
+    if (rec) rec->assign((size_t)C * F, 0.0f);
     std::vector<int> keys;
     for (int c = 0; c < C; ++c) {
         const float* s = &series[(size_t)c * F];
@@ -562,6 +582,7 @@ void build_stream(const std::vector<float>& series, int C, int F, int B, int cod
             }
         }
         append_control(st, s, keys, F, B, codec);
+        if (rec) reconstruct(s, keys, F, codec, rec->data() + (size_t)c * F);
     }
 }
 
@@ -599,9 +620,7 @@ int encode_render_leaf(const Field& full, const vbts_encode_opts& o, uint32_t le
         }
     }
     int mode = 1; // GRID4
-    std::vector<float> fine;
     std::vector<uint16_t> shellVox;
-    std::vector<float> shellSeries;
     if (p.kind == VBTS_LEVELSET) {
         double minAbs = 1e30;
         for (int t = 0; t < F; ++t)
@@ -616,20 +635,7 @@ int encode_render_leaf(const Field& full, const vbts_encode_opts& o, uint32_t le
                         if (occ) shellVox.push_back((uint16_t)((z * 8 + y) * 8 + x));
                     }
             std::sort(shellVox.begin(), shellVox.end());
-            if (!shellVox.empty()) {
-                mode = 2;
-                shellSeries.resize(shellVox.size() * (size_t)F);
-                std::vector<float> g(64);
-                for (int t = 0; t < F; ++t) {
-                    for (int c = 0; c < 64; ++c) g[c] = coarse[(size_t)c * F + t];
-                    for (size_t i = 0; i < shellVox.size(); ++i) {
-                        const int v = shellVox[i];
-                        const int x = v & 7, y = (v >> 3) & 7, z = v >> 6;
-                        shellSeries[i * F + t] =
-                            (float)(fd.eval(x0 + x, y0 + y, z0 + z, t, true) - trilinear_lattice(g.data(), 4, x, y, z));
-                    }
-                }
-            }
+            if (!shellVox.empty()) mode = 2;
         }
     } else {
         // route probe: coarse-grid miss on the 6^3 lattice every 4th frame
@@ -645,30 +651,65 @@ int encode_render_leaf(const Field& full, const vbts_encode_opts& o, uint32_t le
             }
         }
         const float scale = p.kind == VBTS_TEMPERATURE ? 1500.0f : 1.0f;
-        if (worst > o.fineThreshold * scale) {
-            mode = 3;
-            fine.resize((size_t)216 * F);
-            for (int t = 0; t < F; ++t) {
-                for (int c = 0; c < 64; ++c) g[c] = coarse[(size_t)c * F + t];
-                for (int j = 0; j < 216; ++j) {
-                    const double x = (j % 6) * sc6, y = ((j / 6) % 6) * sc6, z = (j / 36) * sc6;
-                    fine[(size_t)j * F + t] =
-                        (float)fd.eval(x0 + x, y0 + y, z0 + z, t, false) - trilinear_lattice(g.data(), 4, x, y, z);
-                }
-            }
-        }
+        if (worst > o.fineThreshold * scale) mode = 3;
     }
     const uint64_t seed = mix64(p.seed ^ ((uint64_t)leaf * 0x2545F4914F6CDD1Dull));
-    for (float epsScale = 1.0f;; epsScale *= 2.0f) {
-        Stream cs, fs;
-        build_stream(coarse, 64, F, 8, o.codec, epsScale, o, seed, cs);
-        if (mode == 3) build_stream(fine, 216, F, 8, o.codec, epsScale, o, seed ^ 1u, fs);
-        if (mode == 2) build_stream(shellSeries, (int)shellVox.size(), F, 4, o.codec, epsScale, o, seed ^ 2u, fs);
-        if ((cs.keyCount > 65535u || fs.keyCount > 65535u) && epsScale < 1e6f) continue;
-        if (cs.keyCount > 65535u || fs.keyCount > 65535u) {
+    // Each stream is rate-controlled on its own (the leaf prefix holds u16
+    // keyframe counts): its error bound grows by 1.25x until it fits.  The
+    // fine / shell residuals are taken against the coarse grid as it decodes
+    // (closed loop), so the coarse keyframe error does not add to theirs.
+    Stream cs, fs;
+    std::vector<float> recCoarse;
+    for (float sc = 1.0f;; sc *= 1.25f) {
+        cs = Stream{};
+        build_stream(coarse, 64, F, 8, o.codec, sc, o, seed, cs, &recCoarse);
+        if (cs.keyCount <= 65535u) break;
+        if (sc > 1e6f) {
             g_err = "leaf keyframe count exceeds u16";
             return 1;
         }
+    }
+    std::vector<float> resid;
+    int residCount = 0;
+    if (mode == 2) {
+        residCount = (int)shellVox.size();
+        resid.resize(shellVox.size() * (size_t)F);
+        std::vector<float> g(64);
+        for (int t = 0; t < F; ++t) {
+            for (int c = 0; c < 64; ++c) g[c] = recCoarse[(size_t)c * F + t];
+            for (size_t i = 0; i < shellVox.size(); ++i) {
+                const int v = shellVox[i];
+                const int x = v & 7, y = (v >> 3) & 7, z = v >> 6;
+                resid[i * F + t] =
+                    (float)(fd.eval(x0 + x, y0 + y, z0 + z, t, true) - trilinear_lattice(g.data(), 4, x, y, z));
+            }
+        }
+    } else if (mode == 3) {
+        residCount = 216;
+        resid.resize((size_t)216 * F);
+        std::vector<float> g(64);
+        const double sc6 = 7.0 / 5.0;
+        for (int t = 0; t < F; ++t) {
+            for (int c = 0; c < 64; ++c) g[c] = recCoarse[(size_t)c * F + t];
+            for (int j = 0; j < 216; ++j) {
+                const double x = (j % 6) * sc6, y = ((j / 6) % 6) * sc6, z = (j / 36) * sc6;
+                resid[(size_t)j * F + t] =
+                    (float)fd.eval(x0 + x, y0 + y, z0 + z, t, false) - trilinear_lattice(g.data(), 4, x, y, z);
+            }
+        }
+    }
+    if (mode != 1) {
+        for (float sc = 1.0f;; sc *= 1.25f) {
+            fs = Stream{};
+            build_stream(resid, residCount, F, mode == 2 ? 4 : 8, o.codec, sc, o, seed ^ (mode == 2 ? 2u : 1u), fs);
+            if (fs.keyCount <= 65535u) break;
+            if (sc > 1e6f) {
+                g_err = "leaf keyframe count exceeds u16";
+                return 1;
+            }
+        }
+    }
+    {
         const uint32_t header = (uint32_t)mode | ((uint32_t)o.codec << 2) | ((uint32_t)o.codec << 4) | (4u << 6) |
                                 ((mode == 3 ? 6u : 0u) << 9);
         put32(out, header);

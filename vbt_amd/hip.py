@@ -234,6 +234,19 @@ class DeviceArray:
         check(lib().vbt_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, self.stream.h if self.stream else None))
         return out
 
+    def host_slice(self, start: int, count: int) -> np.ndarray:
+        """Elements [start, start + count) of the flattened array."""
+        n = int(np.prod(self.shape))
+        if start < 0 or count < 0 or start + count > n:
+            raise ValueError("slice outside the array")
+        out = np.empty(count, self.dtype)
+        src = c_void_p(self.ptr.value + start * self.dtype.itemsize)
+        check(lib().vbt_memcpy_d2h(out.ctypes.data, src, count * self.dtype.itemsize,
+                                   self.stream.h if self.stream else None))
+        if self.stream:
+            self.stream.sync()
+        return out
+
     def fill_bytes(self, value: int):
         check(lib().vbt_memset_d(self.ptr, value, self.nbytes, self.stream.h if self.stream else None))
 
