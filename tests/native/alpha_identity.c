@@ -1,7 +1,10 @@
 /* Exhaustive check of the frame-cache sweep's lerp weight: for every pair of
- * integers 0 < a < b < 2^16 (b - a frames into a keyframe gap of b frames),
- * RN32(RN64(a * RN64(1/b))) must equal RN32(a / b) bit for bit.  Compiled and
- * run by tests/test_cpu_units.py. */
+ * integers 0 < a < b < 2^16 (a frames into a keyframe gap of b frames), with
+ * y = RN32(1/b), q = RN32(a y), r = fma(-b, q, a) and w = fma(r, y, q)
+ * (Markstein's correction in float arithmetic), w must equal RN32(a / b) bit
+ * for bit.  The device forms y from its reciprocal instruction and checks it
+ * against RN32(1/b) itself.  Compiled and run by tests/test_cpu_units.py. */
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -11,14 +14,18 @@ int main(void)
     uint64_t bad = 0, checked = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad, checked)
     for (int b = 2; b < 65536; ++b) {
-        const double r = 1.0 / (double)b;
+        const float fb = (float)b;
+        const float y = 1.0f / fb;
         for (int a = 1; a < b; ++a) {
-            const float q = (float)a / (float)b;
-            const float p = (float)((double)a * r);
-            uint32_t qb, pb;
-            memcpy(&qb, &q, 4);
-            memcpy(&pb, &p, 4);
-            bad += qb != pb;
+            const float fa = (float)a;
+            const float want = fa / fb;
+            const float q = fa * y;
+            const float r = fmaf(-fb, q, fa);
+            const float w = fmaf(r, y, q);
+            uint32_t wb, gb;
+            memcpy(&wb, &want, 4);
+            memcpy(&gb, &w, 4);
+            bad += wb != gb;
             ++checked;
         }
     }

@@ -51,7 +51,7 @@ def test_half_conversions_against_reference():
 def test_lerp_weight_identity_exhaustive(tmp_path):
     exe = str(tmp_path / "alpha_identity")
     subprocess.run(["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-o", exe,
-                    os.path.join(ROOT, "tests", "native", "alpha_identity.c")], check=True)
+                    os.path.join(ROOT, "tests", "native", "alpha_identity.c"), "-lm"], check=True)
     r = subprocess.run([exe], capture_output=True, text=True)
     checked, bad = map(int, r.stdout.split())
     assert r.returncode == 0 and bad == 0 and checked == 2147385345

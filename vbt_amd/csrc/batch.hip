@@ -170,7 +170,7 @@ __device__ inline uint32_t resolve(const BatchArgs& A, uint32_t& q, uint32_t& tr
 template <uint32_t BLOCK, uint32_t WINDOW>
 __global__ void __launch_bounds__(BLOCK) k_cache_batch(BatchArgs A)
 {
-    __shared__ uint32_t lds[WINDOW / 4 + 1]; // + 1: see lds_key
+    __shared__ uint32_t lds[WINDOW / 4 + 4]; // + 4: reads past the window, see sweep_frames
     __shared__ uint32_t sRed[BLOCK / 64];
     __shared__ uint32_t sItem, sEnd;
     const uint32_t tid = threadIdx.x;
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(BLOCK) k_cache_batch(BatchArgs A)
 template <uint32_t BLOCK, uint32_t WINDOW>
 __global__ void __launch_bounds__(BLOCK) k_cache_batch_static(BatchArgs A)
 {
-    __shared__ uint32_t lds[WINDOW / 4 + 1]; // + 1: see lds_key
+    __shared__ uint32_t lds[WINDOW / 4 + 4]; // + 4: reads past the window, see sweep_frames
     __shared__ uint32_t sRed[BLOCK / 64];
     const uint32_t first = blockIdx.x * A.chunk;
     const uint32_t end = min(first + A.chunk, A.items);

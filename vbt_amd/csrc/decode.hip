@@ -352,7 +352,7 @@ template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
 __global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32_t* groupLeaf, int t0, int nt,
                                                         float* out, uint64_t total, uint32_t* errFlag)
 {
-    __shared__ uint32_t lds[WINDOW / 4 + 1]; // + 1: see lds_key
+    __shared__ uint32_t lds[WINDOW / 4 + 4]; // + 4: reads past the window, see sweep_frames
     __shared__ uint32_t sRed[BLOCK / 64];
     cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed);
 }
@@ -661,17 +661,23 @@ __global__ void __launch_bounds__(256) k_fill16(uint4* dst, uint4 v, uint64_t n)
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) dst[i] = v;
 }
 
-// Every lerp weight the sweep can form: block g-1 checks gap g against the
-// correctly rounded double quotient rounded to float (equal to the correctly
-// rounded float quotient for these operands; see sweep_alpha).
+// Every lerp weight the sweep can form: block g-1 checks gap g, its
+// reciprocal against RN32(1 / g) and each weight a / g, 0 <= a < g, against
+// RN32(a / g).  The expected values are double quotients rounded to float:
+// a quotient a / g with g < 2^16 that is not itself a float lies at least
+// 2^-41 (relative) from every float rounding boundary (a boundary is a 25-bit
+// odd multiple of a power of two, and |a / g - m 2^e| >= 2^e / g), far beyond
+// the double rounding error, so rounding twice gives the correctly rounded
+// float quotient.
 __global__ void __launch_bounds__(256) k_selftest_weights(uint32_t maxGap, unsigned long long* mismatches)
 {
     const uint32_t g = blockIdx.x + 1u;
     if (g > maxGap) return;
-    const double r = sweep_recip((int)g);
-    uint32_t bad = 0;
+    const float b = (float)g;
+    const float y = sweep_recip(b);
+    uint32_t bad = threadIdx.x == 0u && __float_as_uint(y) != __float_as_uint((float)(1.0 / (double)g)) ? 1u : 0u;
     for (uint32_t a = threadIdx.x; a < g; a += 256u) {
-        const float got = sweep_alpha((int)a, r);
+        const float got = sweep_alpha((int)a, b, y);
         const float exp = (float)((double)a / (double)g);
         bad += __float_as_uint(got) != __float_as_uint(exp) ? 1u : 0u;
     }

@@ -352,15 +352,21 @@ __device__ inline float cache_slot_exact(const AssetDev& a, uint64_t g, int t, i
 
 // ------------------------------------------------- packed cache: render tile --
 // Lerp weight of the sweep: alpha = (frame - f0) / (f1 - f0) for integer
-// frames 0 <= frame - f0 < f1 - f0 < 2^16, computed as RN32(RN64((frame - f0)
-// * r)) with r within 2^-50 (relative) of 1 / (f1 - f0).  This equals the
-// correctly rounded float quotient the reference computes: the double product
-// is within 2^-49 (relative) of the true quotient, while a quotient a/b with
-// b < 2^16 that is not itself a float lies at least 2^-41 (relative) away from
-// every float rounding boundary (a boundary is a 25-bit odd multiple of a
-// power of two; |a/b - m 2^e| >= 2^e / b), and one that is a float is
-// reproduced exactly.
-__device__ inline float sweep_alpha(int num, double rinv) { return (float)((double)num * rinv); }
+// frames 0 <= a = frame - f0 < b = f1 - f0 < 2^16, equal to the correctly
+// rounded float quotient RN32(a / b) the reference computes.  Given y =
+// RN32(1 / b), Markstein's correction (P. Markstein, IBM J. Res. Dev. 34(1),
+// 1990) yields it in float arithmetic: q = RN32(a y) is within one ulp of
+// a / b, the residual r = a - b q is exact in one FMA, and RN32(q + r y) =
+// RN32(a / b).  b and a are exact floats (< 2^24).  Checked for every pair
+// with y from sweep_recip on the device (vbt_selftest_weights) and with y =
+// RN32(1 / b) on the host (tests/native/alpha_identity.c).
+__device__ __forceinline__ float sweep_alpha(int num, float b, float y)
+{
+    const float a = (float)num;
+    const float q = a * y;
+    const float r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
 
 // Branch-free form of half_to_float for the sweep: the hardware conversion is
 // exact for zeros, subnormals, normals and infinities; NaNs take the
@@ -391,21 +397,18 @@ struct SweepState {
     uint32_t k;
     int fk, fn;
     float vk, vn;
-    double rinv;
+    float gap, y; // fn - fk and RN32(1 / (fn - fk))
 };
 
-// 1 / gap for a key gap 1 <= gap < 2^16 (0 is mapped to 1: such a weight is
-// never used): the hardware double reciprocal refined by two Newton steps,
-// well inside the 2^-50 the weight needs (vbt_selftest_weights checks every
-// gap against a correctly rounded division).
-__device__ __forceinline__ double sweep_recip(int gap)
+// RN32(1 / b) for an integer key gap 1 <= b < 2^16: the hardware float
+// reciprocal (within one ulp) refined by one Newton step with FMAs.  The
+// result is correctly rounded for every such b (vbt_selftest_weights checks
+// each one); a gap of 0 (never used for a weight) gives a non-finite value.
+__device__ __forceinline__ float sweep_recip(float b)
 {
-    const double b = (double)(gap > 1 ? gap : 1);
-    double r = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-b, r, 1.0);
-    return __builtin_fma(r, e, r);
+    const float y = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y, 1.0f);
+    return __builtin_fmaf(e, y, y);
 }
 
 typedef __attribute__((address_space(3))) const uint8_t lds_byte;
@@ -414,8 +417,9 @@ typedef __attribute__((address_space(3))) const uint8_t lds_byte;
 // read with aligned dword loads: an unaligned 16-bit LDS read stalls the LDS
 // pipeline (the staged pool's 3-byte fp16 records sit at every byte offset),
 // so the record is cut out of the two dwords covering it.  The second dword
-// is only needed when the record crosses into it; the LDS array has one
-// padding dword after the window so the read stays inside it.  fp32 records
+// is only needed when the record crosses into it; the LDS arrays have
+// padding dwords after the window (sweep_frames reads past a control's last
+// record) so every read stays inside them.  fp32 records
 // (8 bytes) are 4-byte aligned in the pool: frame in the low half of the
 // first dword, value in the second.
 // The two dwords holding the record at byte offset off, and the record
@@ -468,7 +472,8 @@ __device__ __forceinline__ void sweep_start(lds_byte* L, uint32_t off, uint32_t 
         s.vk = s.vn;
         if (s.k < last) lds_key<KB, NANFIX>(L, off + (s.k + 1u) * KB, s.fn, s.vn);
     }
-    s.rinv = s.k < last ? sweep_recip(s.fn - s.fk) : 0.0;
+    s.gap = (float)(s.fn - s.fk);
+    s.y = sweep_recip(s.gap);
 }
 
 // Raw buffer descriptor of one frame of the output (base, size in bytes):
@@ -485,10 +490,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(char* base, uint32_
 // the same for every lane: a buffer store with the frame's base in scalar
 // registers and the lane's 32-bit offset (strideBytes < 2^32, dstBytes <
 // strideBytes; the host checks the frame size).  The state update is
-// branch-free: the key after next (record min(k + 2, last), tracked as a byte
-// offset) is read every step and selected on advance.  Step i issues that LDS
-// read first and produces frame i - 1's output (saved by step i - 1) while
-// the read is in flight, then updates the state for frame i.
+// branch-free: the key after next (record k + 2, tracked as a byte offset) is
+// read every step and selected on advance.  Step i issues that LDS read first
+// and produces frame i - 1's output (saved by step i - 1) while the read is
+// in flight, then updates the state for frame i.  Once k + 2 passes the last
+// record the read returns whatever follows it in LDS (the next control's
+// records or padding); such a record only becomes key k + 1 when k reaches
+// the last key, where the output is vk and never uses it.  The read reaches
+// at most KB + 7 bytes past the window's staged records: record k + 2 <=
+// last + 2 starts at most KB bytes past the control's end, which is inside
+// the window, and the read covers two dwords from there.
 template <uint32_t KB, bool NANFIX>
 __device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t last, const SweepState& s,
                                              int tBegin, int cn, char* out, uint32_t dstBytes, uint64_t strideBytes)
@@ -496,40 +507,42 @@ __device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t
     uint32_t k = s.k;
     int fk = s.fk, fn = s.fn;
     float vk = s.vk, vn = s.vn;
-    double rinv = s.rinv;
-    const uint32_t lastOff = off + last * KB;
-    // byte offset of record min(k + 2, last); the record is carried raw (the
-    // LDS read in flight) and decoded where the next step needs it
-    uint32_t pOff = min(off + (k + 2u) * KB, lastOff);
+    float gap = s.gap, y = s.y;
+    // byte offset of record k + 2; the record is carried raw (the LDS read in
+    // flight) and decoded where the next step needs it
+    uint32_t pOff = off + (k + 2u) * KB;
     RawKey raw = lds_fetch(L, pOff);
     // step i's values for frame i's output, produced in step i + 1
     int fkS = 0;
     float vkS = 0.0f, vnS = 0.0f;
     bool endS = false;
-    double rinvS = 0.0;
+    float gapS = 1.0f, yS = 1.0f;
     auto update = [&](int frame) {
         int fp;
         float vp;
         key_decode<KB, NANFIX>(raw, fp, vp);
         const bool adv = (k < last) & (fn <= frame);
         const uint32_t step = adv ? 1u : 0u;
-        const double rNew = sweep_recip(fp - fn);
+        const float gNew = (float)(fp - fn);
+        const float yNew = sweep_recip(gNew);
         fk = adv ? fn : fk;
         vk = adv ? vn : vk;
         fn = adv ? fp : fn;
         vn = adv ? vp : vn;
         k += step;
-        rinv = adv ? rNew : rinv;
-        pOff = min(pOff + step * KB, lastOff);
+        gap = adv ? gNew : gap;
+        y = adv ? yNew : y;
+        pOff += step * KB;
         raw = lds_fetch(L, pOff);
         fkS = fk;
         vkS = vk;
         vnS = vn;
         endS = k == last;
-        rinvS = rinv;
+        gapS = gap;
+        yS = y;
     };
     auto output = [&](int frame) {
-        const float alpha = sweep_alpha(frame - fkS, rinvS);
+        const float alpha = sweep_alpha(frame - fkS, gapS, yS);
         const float lerp = vkS + alpha * (vnS - vkS);
         const float val = (fkS >= frame || endS) ? vkS : lerp;
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(val), frame_rsrc(out, (uint32_t)strideBytes),
