@@ -100,6 +100,62 @@ def test_unsorted_keyframes_take_exact_path():
     a.close()
 
 
+def shuffle_keyframes(data: bytes, o, seed: int) -> bytes:
+    """Permute the keyframe records of every multi-key coarse control of every
+    2nd active render leaf (seeded): bin hints then point past or short of
+    the frame and the binary search runs over unsorted frames."""
+    rng = np.random.default_rng(seed)
+    buf = bytearray(data)
+    base = 52 + 4 * o.header.coarseAcScaleCount + 4 * (o.header.leafCount + 1)
+    compact, _ = o.page_table()
+    for leaf in np.nonzero(compact != 0xFFFFFFFF)[0][::2]:
+        v = o.parse(int(leaf))
+        lb = base + 4 * int(o.offsets[leaf])
+        kb = 8 if v.header.coarseCodec == 1 else 3
+        key0 = lb + v.layout.coarse.keyframeOffset
+        for c in range(v.header.coarseResolution ** 3):
+            ks, kc = struct.unpack_from("<HB", buf, lb + 8 + 4 * c)
+            if kc >= 2:
+                a0 = key0 + kb * ks
+                recs = [bytes(buf[a0 + kb * i:a0 + kb * (i + 1)]) for i in range(kc)]
+                buf[a0:a0 + kb * kc] = b"".join(recs[i] for i in rng.permutation(kc))
+    return bytes(buf)
+
+
+@pytest.mark.parametrize("name", ["density_fp16", "density_fp32_bursts", "flames"])
+def test_unsorted_keyframes_queries_and_short_caches(name):
+    """Batched and plain point queries and short frame caches equal the
+    oracle (the reference's bracket search) on streams whose frames are
+    permuted."""
+    data = SMALL[name].encode(threads=8)
+    bad = shuffle_keyframes(data, oracle_py.OracleFile(data), 7)
+    a = hip.Asset(bad)
+    o = oracle_py.OracleFile(bad)
+    assert a.info.fast_leaves < a.info.activeLeaves
+    W, H, D, F = a.dims
+    compact, _ = o.page_table()
+    act = np.nonzero(compact != 0xFFFFFFFF)[0]
+    rng = np.random.default_rng(11)
+    n = 30000
+    leaves = act[rng.integers(0, len(act), n)]
+    lcx, lcy = (W + 7) // 8, (H + 7) // 8
+    q = np.stack([np.minimum((leaves % lcx) * 8 + rng.integers(0, 8, n), W - 1),
+                  np.minimum(((leaves // lcx) % lcy) * 8 + rng.integers(0, 8, n), H - 1),
+                  np.minimum((leaves // (lcx * lcy)) * 8 + rng.integers(0, 8, n), D - 1),
+                  rng.integers(0, F + 3, n)], axis=1).astype(np.uint32)
+    exp = o.query(q)
+    bitexact(a.query(q), exp, f"{name} shuffled queries")
+    bitexact(a.query(q, plain=True), exp, f"{name} shuffled plain queries")
+    bitexact(a.dense(F // 2), o.frame(F // 2), f"{name} shuffled dense")
+    # one- and two-frame caches take the per-slot exact kernel, longer runs the
+    # sweep with its exact fallback for these leaves
+    for t0, nt in ((0, 1), (F // 3, 1), (F - 2, 2), (1, 5)):
+        got = a.frame_cache(t0, nt)
+        for i in range(nt):
+            bitexact(got[i], o.cache_packed(t0 + i)[0], f"{name} shuffled cache t={t0 + i} (nt={nt})")
+    a.close()
+
+
 def test_corrupt_leaf_fails_alone():
     data = bytearray(SMALL["levelset"].encode(threads=8))
     o = oracle_py.OracleFile(bytes(data))

@@ -386,13 +386,8 @@ __global__ void __launch_bounds__(256) k_cache_exact(AssetDev a, int t0, int nt,
     }
     const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
     const uint32_t ci = isCoarse ? slot : slot - d.cc;
-    const Bytes b{a.payload + (size_t)d.word * 4u};
-    const uint32_t desc = b.u32(s.base + ci * 4u);
-    const uint32_t keyStart = desc & 0xFFFFu, keyCount = (desc >> 16) & 0xFFu;
-    const uint32_t binBase = s.base + s.count * 4u + ci * s.bins;
-    for (int i = 0; i < nt; ++i)
-        o[(uint64_t)i * stride] =
-            decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins, (int)a.F, t0 + i);
+    const uint32_t* W = (const uint32_t*)(a.payload + (size_t)d.word * 4u);
+    for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = decode_control_w(W, s, ci, (int)a.F, t0 + i);
 }
 
 template <int LAYOUT>

@@ -331,6 +331,86 @@ __device__ inline float point_query(const AssetDev& a, const ComposeDev& cm, uin
     return render_point<LANES>(a, d, cm, r.x & 7u, r.y & 7u, r.z & 7u, t, sub);
 }
 
+// decode_control (decodeFromSequence, src/render_temporal_decode.cpp:79-131:
+// bin hint, fallbacks, early outs, binary search, v0 + alpha * (v1 - v0))
+// with aligned dword loads from the leaf base W: the descriptor and the two
+// bin-hint bytes are read together, each keyframe record (frame and value)
+// with one pair of dwords.  Same result as decode_control for every stream;
+// reads stay inside the leaf's validated stream plus 4 bytes (the payload
+// allocation is padded).
+__device__ __forceinline__ uint32_t w_u32(const uint32_t* W, uint32_t byteOff)
+{
+    const uint32_t i = byteOff >> 2;
+    return __builtin_amdgcn_alignbyte(W[i + 1u], W[i], byteOff & 3u);
+}
+
+__device__ __forceinline__ void w_rec(const uint32_t* W, uint32_t off, uint32_t codec, int& f, float& v)
+{
+    const uint32_t i = off >> 2, lo = W[i], hi = W[i + 1u];
+    if (codec == CODEC_FP32) {
+        f = (int)(lo & 0xFFFFu); // fp32 records are 4-byte aligned
+        v = __uint_as_float(hi);
+    } else {
+        const uint32_t x = __builtin_amdgcn_alignbyte(hi, lo, off & 3u);
+        f = (int)(x & 0xFFu);
+        v = half_to_float((x >> 8) & 0xFFFFu);
+    }
+}
+
+__device__ inline float decode_control_w(const uint32_t* W, const StreamRef& s, uint32_t ci, int F, int t)
+{
+    const int frame = clamp_frame(t, F);
+    const uint32_t B = s.bins;
+    const uint32_t q = ((uint32_t)frame * B) / (uint32_t)(F > 1 ? F : 1);
+    const uint32_t bin = B != 0u ? (q < B - 1u ? q : B - 1u) : 0u;
+    const uint32_t desc = W[(s.base + ci * 4u) >> 2]; // descriptors are 4-byte aligned
+    const uint32_t hw = w_u32(W, s.base + s.count * 4u + ci * B + bin);
+    const uint32_t keyStart = desc & 0xFFFFu, keyCount = (desc >> 16) & 0xFFu;
+    if (keyCount == 0u) return 0.0f;
+    const uint32_t last = keyCount - 1u, kb = keyframe_bytes(s.codec);
+    const uint32_t k0 = s.keyBase + keyStart * kb;
+    uint32_t lo = 0u;
+    if (B != 0u) lo = (hw & 0xFFu) < last ? (hw & 0xFFu) : last;
+    int f0, f1;
+    float v0, v1;
+    w_rec(W, k0 + lo * kb, s.codec, f0, v0);
+    if (f0 > frame) {
+        lo = 0u;
+        w_rec(W, k0, s.codec, f0, v0);
+    }
+    if (f0 >= frame || lo == last) return v0;
+    uint32_t hi = last;
+    if (bin + 1u < B) {
+        const uint32_t h = ((hw >> 8) & 0xFFu) + 1u;
+        hi = h < last ? h : last;
+    }
+    if (hi < lo + 1u) hi = lo + 1u;
+    w_rec(W, k0 + hi * kb, s.codec, f1, v1);
+    if (f1 < frame) {
+        hi = last;
+        w_rec(W, k0 + hi * kb, s.codec, f1, v1);
+    }
+    if (f1 <= frame) return v1;
+    while (hi > lo + 1u) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        int fm;
+        float vm;
+        w_rec(W, k0 + mid * kb, s.codec, fm, vm);
+        if (fm < frame) {
+            lo = mid;
+            f0 = fm;
+            v0 = vm;
+        } else {
+            hi = mid;
+            f1 = fm;
+            v1 = vm;
+        }
+    }
+    if (f1 == frame) return v1;
+    const float alpha = (float)(frame - f0) / (float)(f1 - f0);
+    return v0 + alpha * (v1 - v0);
+}
+
 // One packed-cache slot of a render asset at one frame (reference bracket
 // search); returns NaN and sets st for a poisoned leaf.
 __device__ inline float cache_slot_exact(const AssetDev& a, uint64_t g, int t, int32_t& st)
@@ -346,8 +426,8 @@ __device__ inline float cache_slot_exact(const AssetDev& a, uint64_t g, int t, i
     const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
     const bool isCoarse = slot < d.cc;
     const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
-    const Bytes b{a.payload + (size_t)d.word * 4u};
-    return decode_control(b, s, isCoarse ? slot : slot - d.cc, (int)a.F, t);
+    return decode_control_w((const uint32_t*)(a.payload + (size_t)d.word * 4u), s, isCoarse ? slot : slot - d.cc,
+                            (int)a.F, t);
 }
 
 // ------------------------------------------------- packed cache: render tile --
