@@ -439,6 +439,40 @@ int vbt_device_fill(void* d_dst, uint32_t word, size_t bytes, void* stream);
  * maxGap = 65535), plus the gaps whose reciprocal is not correctly rounded;
  * expected 0.  Synchronises `stream`. */
 int vbt_selftest_weights(uint32_t maxGap, uint64_t* mismatches, void* stream);
+/* ---- sampling and ray marching over a frame cache --------------------------
+ * The reference viewer's sampler (VBTStudio/frontend/shaders/vbt_sample.comp:
+ * sampleVbtValue :322-415, intersectWorldBox :417-428, traceShadow :430-452,
+ * estimateThickness :454-474, traceLevelSet :476-551, main :553-622) reading
+ * a one-frame cache that vbt_decode_frame_cache built in the
+ * VBT_CACHE_STRIDE576 layout (activeLeaves * 576 floats).  Render assets
+ * (profile 1) whose leaves all validate.  Operations the shader leaves
+ * implementation-defined are fixed: mix(x, y, a) = x*(1-a) + y*a, round()
+ * half away from zero, min(x, y) = y < x ? y : x, max(x, y) = x < y ? y : x,
+ * length = sqrt((x*x + y*y) + z*z); IEEE float without contraction.  The
+ * parameter block mirrors the shader's push constants (vbt_sample.comp:
+ * 38-65) that are not image or volume sizes. */
+typedef struct {
+    uint32_t stepCount;   /* march steps per ray (pc.stepCount) */
+    int32_t frameIndex;   /* frame the cache holds; coarse grids above 64 controls decode from the payload at it */
+    float bboxMin[3];     /* world position of voxel (0, 0, 0) */
+    uint32_t renderMode;  /* 0 density integration, 1 level-set surface */
+    float voxelSize, backgroundValue, surfaceIso, surfaceEpsilonVoxels, surfaceNormalStep, maxStepVoxels;
+    uint32_t surfaceModel; /* density: 8 depth bins; level set: thickness and floor shadow */
+    float floorHeight;
+    uint32_t shadowSteps;
+    uint32_t upAxis;      /* 2: z up, else y up (pc.padding) */
+} vbt_march_params;
+
+/* sampleVbtValue at n world points (3 floats each, device memory). */
+int vbt_sample_points(vbt_asset a, const float* d_cache576, const vbt_march_params* p, const float* d_xyz,
+                      uint32_t n, int preserveInterior, float* d_out, void* stream);
+/* The shader's main for n rays: d_rays holds 8 floats per ray (origin xyz,
+ * tmin, direction xyz, tmax), d_out 8 per ray (primary, auxiliary: density
+ * sum or 8 depth bins; level set: normal, hit t, thickness, floor flag,
+ * shadow, |n.-d|); both 16-byte aligned. */
+int vbt_march_rays(vbt_asset a, const float* d_cache576, const vbt_march_params* p, const float* d_rays, uint32_t n,
+                   float* d_out, void* stream);
+
 int vbt_stream_create(void** stream);
 int vbt_stream_destroy(void* stream);
 int vbt_stream_synchronize(void* stream);

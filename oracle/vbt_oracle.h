@@ -183,6 +183,29 @@ int vbto_frame_cache_576(const vbto_file* f, int frameIndex, float* out);
 /* Packed cache: out[ctrlOffset[compact] + slot] with the meaningful counts only. */
 int vbto_frame_cache_packed(const vbto_file* f, int frameIndex, const uint64_t* ctrlOffset, float* out);
 
+/* ---- viewer sampling and ray marching over a stride-576 frame cache
+ *      (VBTStudio/frontend/shaders/vbt_sample.comp, restated in
+ *      vbt_oracle_march.c; render assets only) ---- */
+typedef struct {
+    uint32_t stepCount;  /* march steps per ray */
+    int32_t frameIndex;  /* frame the cache holds (direct decode of coarse grids > 64 controls) */
+    float bboxMin[3];    /* world position of voxel (0, 0, 0) */
+    uint32_t renderMode; /* 0 density integration, 1 level-set surface */
+    float voxelSize, backgroundValue, surfaceIso, surfaceEpsilonVoxels, surfaceNormalStep, maxStepVoxels;
+    uint32_t surfaceModel; /* density: 8 depth bins; level set: thickness and floor shadow */
+    float floorHeight;
+    uint32_t shadowSteps;
+    uint32_t upAxis;     /* 2: z up, else y up (the shader's pc.padding) */
+} vbto_march_params;
+
+/* sampleVbtValue at n world points xyz (3 floats each). */
+int vbto_sample_points(const vbto_file* f, const float* cache576, const vbto_march_params* p, const float* xyz,
+                       uint64_t n, int preserveInterior, float* out, int nthreads);
+/* The shader's main over n rays (8 floats each: origin, tmin, direction,
+ * tmax); out gets 8 floats per ray (primary, auxiliary). */
+int vbto_march_rays(const vbto_file* f, const float* cache576, const vbto_march_params* p, const float* rays,
+                    uint64_t n, float* out, int nthreads);
+
 /* ---- scientific route  (render/src/vbt_file.cpp:220-248, render/src/scientific_decode.cpp) ---- */
 typedef struct {
     uint32_t mode, coarseKeepMinus1, coarseQuantClass, tierId, framesPerBinMinus1, eventQuantCode,

@@ -106,6 +106,10 @@ def oracle():
         L.vbto_leaf_cache_count.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32]
         L.vbto_frame_cache_576.argtypes = [ctypes.POINTER(VFile), ctypes.c_int, ctypes.c_void_p]
         L.vbto_frame_cache_packed.argtypes = [ctypes.POINTER(VFile), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.vbto_sample_points.argtypes = [ctypes.POINTER(VFile), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+        L.vbto_march_rays.argtypes = [ctypes.POINTER(VFile), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int]
         L.vbto_dct_basis.restype = ctypes.c_float
         L.vbto_dct_basis.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.vbto_sci_value_at.argtypes = [ctypes.POINTER(VFile)] + [ctypes.c_uint32] * 5 + [ctypes.POINTER(ctypes.c_float)]
@@ -227,6 +231,29 @@ class OracleFile:
         if rc != 0:
             raise ValueError(last_error())
         return out.reshape(int(n), 576)
+
+    def sample_points(self, cache576: np.ndarray, params, xyz: np.ndarray, preserve_interior=False,
+                      threads: int = 8) -> np.ndarray:
+        """sampleVbtValue (vbt_oracle_march.c); params is a vbt_march_params
+        ctypes structure (same layout as the oracle's)."""
+        cache = np.ascontiguousarray(cache576, np.float32)
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        out = np.empty(len(xyz), np.float32)
+        rc = oracle().vbto_sample_points(ctypes.byref(self.vf), cache.ctypes.data, ctypes.byref(params),
+                                         xyz.ctypes.data, len(xyz), int(preserve_interior), out.ctypes.data, threads)
+        if rc != 0:
+            raise ValueError(last_error())
+        return out
+
+    def march_rays(self, cache576: np.ndarray, params, rays: np.ndarray, threads: int = 8) -> np.ndarray:
+        cache = np.ascontiguousarray(cache576, np.float32)
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.empty((len(rays), 8), np.float32)
+        rc = oracle().vbto_march_rays(ctypes.byref(self.vf), cache.ctypes.data, ctypes.byref(params), rays.ctypes.data,
+                                      len(rays), out.ctypes.data, threads)
+        if rc != 0:
+            raise ValueError(last_error())
+        return out
 
     def parse(self, leaf: int) -> LeafView:
         b = self.leaf_bytes(leaf)

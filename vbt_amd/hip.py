@@ -50,6 +50,32 @@ class Compose(ctypes.Structure):
                 ("shellEmptyMax", ctypes.c_float)]
 
 
+class MarchParams(ctypes.Structure):
+    """vbt_march_params (the sampler's push constants)."""
+    _fields_ = [("stepCount", c_u32), ("frameIndex", ctypes.c_int32), ("bboxMin", ctypes.c_float * 3),
+                ("renderMode", c_u32), ("voxelSize", ctypes.c_float), ("backgroundValue", ctypes.c_float),
+                ("surfaceIso", ctypes.c_float), ("surfaceEpsilonVoxels", ctypes.c_float),
+                ("surfaceNormalStep", ctypes.c_float), ("maxStepVoxels", ctypes.c_float), ("surfaceModel", c_u32),
+                ("floorHeight", ctypes.c_float), ("shadowSteps", c_u32), ("upAxis", c_u32)]
+
+
+def march_params(**kw) -> MarchParams:
+    """Defaults: 128 steps, frame 0, origin at 0, density mode, unit voxels,
+    background 0, iso 0, epsilon 0.5 voxel, normal step 0.5, max step 4,
+    no surface model, floor at 0, 64 shadow steps, y up."""
+    p = MarchParams(stepCount=128, frameIndex=0, renderMode=0, voxelSize=1.0, backgroundValue=0.0, surfaceIso=0.0,
+                    surfaceEpsilonVoxels=0.5, surfaceNormalStep=0.5, maxStepVoxels=4.0, surfaceModel=0,
+                    floorHeight=0.0, shadowSteps=64, upAxis=1)
+    for k, v in kw.items():
+        if k == "bboxMin":
+            p.bboxMin[:] = [float(x) for x in v]
+        elif hasattr(p, k):
+            setattr(p, k, v)
+        else:
+            raise TypeError(f"unknown march parameter {k}")
+    return p
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbmBudgetBytes", c_u64), ("arenaBytes", c_u64), ("stagingBytes", c_u64),
                 ("maxInflight", c_u32), ("blocksPerCU", c_u32), ("queryItem", c_u32), ("cacheItem", c_u32),
@@ -128,6 +154,10 @@ def lib():
         L.vbt_query_points.argtypes = [c_void_p, c_void_p, c_u32, ctypes.POINTER(Compose), c_void_p, c_void_p,
                                        c_void_p]
         L.vbt_query_points_plain.argtypes = L.vbt_query_points.argtypes
+        L.vbt_sample_points.argtypes = [c_void_p, c_void_p, ctypes.POINTER(MarchParams), c_void_p, c_u32, c_int,
+                                        c_void_p, c_void_p]
+        L.vbt_march_rays.argtypes = [c_void_p, c_void_p, ctypes.POINTER(MarchParams), c_void_p, c_u32, c_void_p,
+                                     c_void_p]
         L.vbt_decode_control_values.argtypes = [c_void_p, c_u32, ctypes.c_uint16, ctypes.c_uint8, ctypes.c_uint8,
                                                 c_u32, c_u32, c_int, c_int, c_void_p, c_void_p, c_u32, c_void_p,
                                                 c_void_p, c_void_p]
@@ -404,6 +434,38 @@ class Asset:
         self.query_into(dq, n, out, comp, st, plain)
         vals = out.host()
         return (vals, st.host()) if with_status else vals
+
+
+    def sample_points(self, cache576: DeviceArray, params: MarchParams, xyz: np.ndarray,
+                      preserve_interior: bool = False) -> np.ndarray:
+        """The viewer's sampleVbtValue at world points (n x 3) from a one-frame
+        stride-576 cache."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        n = len(xyz)
+        if n == 0:
+            return np.zeros(0, np.float32)
+        d = DeviceArray.from_host(xyz, self.stream)
+        out = DeviceArray(n, np.float32, self.stream)
+        check(lib().vbt_sample_points(self.h, cache576.ptr, ctypes.byref(params), d.ptr, n, int(preserve_interior),
+                                      out.ptr, self.stream.h))
+        return out.host()
+
+    def march_into(self, cache576: DeviceArray, params: MarchParams, rays: DeviceArray, n: int, out: DeviceArray,
+                   stream: Stream | None = None):
+        check(lib().vbt_march_rays(self.h, cache576.ptr, ctypes.byref(params), rays.ptr, n, out.ptr,
+                                   (stream or self.stream).h))
+
+    def march(self, cache576: DeviceArray, params: MarchParams, rays: np.ndarray) -> np.ndarray:
+        """The viewer's ray march (n x 8 rays: origin, tmin, direction, tmax)
+        -> n x 8 results (primary, auxiliary)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = len(rays)
+        if n == 0:
+            return np.zeros((0, 8), np.float32)
+        d = DeviceArray.from_host(rays, self.stream)
+        out = DeviceArray((n, 8), np.float32, self.stream)
+        self.march_into(cache576, params, d, n, out)
+        return out.host()
 
 
 def status_message(code: int) -> str:
