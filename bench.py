@@ -253,6 +253,92 @@ def serving_bench(hip, loadgen, assets: dict, requests: int, seed: int = 0x5EED0
     }
 
 
+def camera_rays(W, H, D, nx, ny, eye_scale, look_y, tmax):
+    """Pinhole camera rays (origin, tmin 0, direction, tmax) looking at the
+    volume from above and to the side, y up."""
+    eye = np.array([W * eye_scale[0], H * eye_scale[1], D * eye_scale[2]])
+    look = np.array([W / 2, H * look_y, D / 2])
+    fwd = (look - eye) / np.linalg.norm(look - eye)
+    right = np.cross(fwd, [0.0, 1.0, 0.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    u, v = np.meshgrid(np.linspace(-0.5, 0.5, nx), np.linspace(-0.5, 0.5, ny))
+    d = fwd[None] + u.reshape(-1, 1) * right[None] + v.reshape(-1, 1) * up[None]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    n = len(d)
+    return np.concatenate([np.repeat(eye[None], n, 0), np.zeros((n, 1)), d, np.full((n, 1), tmax)],
+                          1).astype(np.float32)
+
+
+def step_points(rays, steps):
+    """Density-mode sample positions of each ray (float64 estimate, for the
+    fraction of samples inside the volume)."""
+    dt = rays[:, 7:8] / max(steps, 1)
+    t = rays[:, 3:4] + 0.5 * dt + dt * np.arange(steps)[None, :]
+    return rays[:, None, :3] + rays[:, None, 4:7] * t[..., None]
+
+
+def march_bench(hip, assets: dict, raws: dict, steps: int, cpu_threads: int, cpu_budget_s: float) -> dict:
+    """The viewer's ray marcher on a one-frame stride-576 cache: a 1024x1024
+    level-set image of liquid320 (frame 75, floor shadows) and a 512x512
+    density image of smoke256 (frame 60, 256 samples per ray), HIP events
+    around the march launches; the oracle's march on the host cores for a
+    bounded sample of the same rays."""
+    out = {}
+    cases = []
+    if "liquid320" in assets:
+        cases.append(("liquid320_levelset_1024", assets["liquid320"], 75, 1024,
+                      dict(renderMode=1, backgroundValue=4.0, stepCount=256, maxStepVoxels=4.0, surfaceModel=1,
+                           floorHeight=0.5, shadowSteps=96, upAxis=1), (1.4, 1.1, -0.6), 0.35, 3.0))
+    if "smoke256" in assets:
+        # camera in front of the z = 0 face; rays span the volume's depth
+        cases.append(("smoke256_density_512", assets["smoke256"], 60, 512,
+                      dict(renderMode=0, stepCount=256, surfaceModel=1), (0.5, 0.5, -0.9), 0.5, 2.0))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    for name, a, t, npx, kw, eye, look_y, reach in cases:
+        W, H, D, F = a.dims
+        st = a.stream
+        cache = hip.DeviceArray((1, a.info.activeLeaves * 576), np.float32, st)
+        a.frame_cache_into(cache, t, 1, hip.CACHE_STRIDE576)
+        rays = camera_rays(W, H, D, npx, npx, eye, look_y, reach * max(W, H, D))
+        n = len(rays)
+        d_rays = hip.DeviceArray.from_host(rays, st)
+        d_out = hip.DeviceArray((n, 8), np.float32, st)
+        p = hip.march_params(frameIndex=t, **kw)
+        ms = timed(hip, st, lambda: a.march_into(cache, p, d_rays, n, d_out), max(3, steps // 4))
+        res = d_out.host()
+        entry = {"rays": n, "ms": round(ms, 3), "rays_per_s": round(n / (ms * 1e-3), 1)}
+        if kw.get("renderMode", 0) == 0:
+            entry["samples_per_s"] = round(n * kw["stepCount"] / (ms * 1e-3), 1)
+            pts = step_points(rays[::97], kw["stepCount"])
+            inside = (pts >= 0).all(-1) & (pts[..., 0] <= W - 1) & (pts[..., 1] <= H - 1) & (pts[..., 2] <= D - 1)
+            entry["samples_inside_fraction"] = round(float(inside.mean()), 4)
+        else:
+            entry["hit_fraction"] = round(float((res[:, 3] >= 0).mean()), 4)
+        if cpu_budget_s > 0:
+            import oracle_py
+            of = oracle_py.OracleFile(raws[name.split("_")[0]]) if name.split("_")[0] in raws else None
+            if of is not None:
+                hc = cache.host().reshape(-1, 576)
+                m = min(n, 8192)
+                sel = rays[np.random.default_rng(1).choice(n, m, replace=False)]
+                t0 = time.perf_counter()
+                k = 0
+                while True:
+                    of.march_rays(hc, p, sel, threads=cpu_threads)
+                    k += 1
+                    if time.perf_counter() - t0 > cpu_budget_s:
+                        break
+                dt = time.perf_counter() - t0
+                entry["cpu_baseline"] = {"rays_per_s": round(k * m / dt, 1), "cores": cpu_threads, "kind": "port",
+                                         "sample": f"{k} x {m} of the same rays in {dt:.1f} s"}
+        out[name] = entry
+        print(f"[bench] march {name}: {n} rays in {ms:.3f} ms = {entry['rays_per_s'] / 1e6:.1f} M rays/s"
+              + (f" (CPU {entry['cpu_baseline']['rays_per_s'] / 1e6:.3f} M rays/s)" if "cpu_baseline" in entry else ""),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +354,7 @@ def main():
     ap.add_argument("--no-queries", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-ceilings", action="store_true")
+    ap.add_argument("--no-march", action="store_true")
     ap.add_argument("--batch-tune-reps", type=int, default=2)
     ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
                     help="assets of the batched frame-cache build (all frames of each, one graph replay)")
@@ -386,12 +473,15 @@ def main():
         "queries": {"random_65536": qres, "random_256": small},
     }
     extra = {args.workload: asset}
+    raws = {args.workload: data}
     want = ([] if args.no_batch else [x for x in args.batch_assets.split(",") if x]) + \
-        ([] if args.no_serve else [x for x in args.serve_assets.split(",") if x])
+        ([] if args.no_serve else [x for x in args.serve_assets.split(",") if x]) + \
+        ([] if args.no_march else ["liquid320"])
     for n in want:
         if n not in extra:
             t0 = time.time()
-            extra[n] = hip.Asset(synth.WORKLOADS[n].encode(threads=min(16, os.cpu_count() or 1)))
+            raws[n] = synth.WORKLOADS[n].encode(threads=min(16, os.cpu_count() or 1))
+            extra[n] = hip.Asset(raws[n])
             print(f"[bench] asset {n}: encoded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     if not args.no_batch:
         line["batch_single"] = batch_bench(hip, extra, [args.workload], args.steps, args.batch_tune_reps)
@@ -402,6 +492,9 @@ def main():
               f"{bb['plain_per_asset']['ms']} ms ({bb['batched_graph']['decoded_gbs']} GB/s decoded), shape "
               f"{bb['shape']}; single {line['batch_single']['batched_graph']['ms']} ms vs one-shot "
               f"{line['batch_single']['plain_per_asset']['ms']} ms", file=sys.stderr, flush=True)
+    if not args.no_march:
+        line["march"] = march_bench(hip, extra, raws, args.steps, min(16, os.cpu_count() or 1),
+                                    0.0 if args.no_cpu else 3.0)
     if not args.no_serve:
         serve = {n: extra[n] for n in [args.workload] + [x for x in args.serve_assets.split(",") if x]}
         line["serving"] = serving_bench(hip, loadgen, serve, args.serve_requests)

@@ -666,8 +666,15 @@ int vbt_cache_batch_autotune(vbt_cache_batch b, int reps, vbt_batch_shape* best,
         }
         return VBT_OK;
     };
-    // coordinate descent: kernel form x tile shape, then frame ranges, chunk,
-    // order, queues (queues only shape the persistent form)
+    auto msOf = [&](const vbt_batch_shape& sh) {
+        for (const auto& t : b->trials)
+            if (!std::memcmp(&t.first, &sh, sizeof(sh))) return t.second;
+        return 1e30f;
+    };
+    // coordinate descent: kernel form x tile shape, then -- from the best tile
+    // of each form, since the two forms often tie on tiles alone -- frame
+    // ranges, chunk, order, queues (persistent form) or a heavy-tile kernel
+    // (static form)
     for (int p : {1, 0})
         for (int t = 0; t < CACHE_TILE_COUNT && !rc; ++t) {
             vbt_batch_shape sh = cur;
@@ -675,52 +682,70 @@ int vbt_cache_batch_autotune(vbt_cache_batch b, int reps, vbt_batch_shape* best,
             sh.tile = t;
             rc = timeShape(sh);
         }
-    cur = top;
-    for (int r : {1, 2, 4, 8}) {
+    for (int form : {1, 0}) {
         if (rc) break;
-        vbt_batch_shape sh = cur;
-        sh.ranges = r;
-        rc = timeShape(sh);
-    }
-    cur = top;
-    for (int c : {1, 2, 4}) {
-        if (rc) break;
-        vbt_batch_shape sh = cur;
-        sh.chunk = c;
-        rc = timeShape(sh);
-    }
-    cur = top;
-    for (int o : {0, 1}) {
-        if (rc) break;
-        vbt_batch_shape sh = cur;
-        sh.order = o;
-        rc = timeShape(sh);
-    }
-    cur = top;
-    for (int q : {1, 8}) {
-        if (rc || !cur.persistent) break;
-        vbt_batch_shape sh = cur;
-        sh.queues = q;
-        rc = timeShape(sh);
-    }
-    // static form: tiles whose records exceed the window go to a second
-    // kernel with a larger window (same slots per workgroup)
-    cur = top;
-    for (int h = 0; h < CACHE_TILE_COUNT && !rc && !cur.persistent; ++h) {
-        if (CACHE_TILES[h].block != CACHE_TILES[cur.tile].block || CACHE_TILES[h].window <= CACHE_TILES[cur.tile].window)
-            continue;
-        vbt_batch_shape sh = cur;
-        sh.heavyTile = h;
-        rc = timeShape(sh);
-    }
-    // with a heavy kernel, a smaller light window may now pay
-    cur = top;
-    for (int t = 0; t < CACHE_TILE_COUNT && !rc && cur.heavyTile >= 0; ++t) {
-        if (CACHE_TILES[t].block != CACHE_TILES[cur.tile].block || CACHE_TILES[t].window >= CACHE_TILES[cur.heavyTile].window)
-            continue;
-        vbt_batch_shape sh = cur;
-        sh.tile = t;
-        rc = timeShape(sh);
+        vbt_batch_shape lt = top;
+        float lms = 1e30f;
+        for (const auto& t : b->trials)
+            if (t.first.persistent == form && t.second < lms) {
+                lms = t.second;
+                lt = t.first;
+            }
+        auto step = [&](const vbt_batch_shape& sh) {
+            if (rc) return;
+            rc = timeShape(sh);
+            const float ms = msOf(sh);
+            if (!rc && ms < lms) {
+                lms = ms;
+                lt = sh;
+            }
+        };
+        vbt_batch_shape cur2 = lt;
+        for (int r : {1, 2, 4, 8}) {
+            vbt_batch_shape sh = cur2;
+            sh.ranges = r;
+            step(sh);
+        }
+        cur2 = lt;
+        for (int c : {1, 2, 4}) {
+            vbt_batch_shape sh = cur2;
+            sh.chunk = c;
+            step(sh);
+        }
+        cur2 = lt;
+        for (int o : {0, 1}) {
+            vbt_batch_shape sh = cur2;
+            sh.order = o;
+            step(sh);
+        }
+        cur2 = lt;
+        for (int q : {1, 8}) {
+            if (!cur2.persistent) break;
+            vbt_batch_shape sh = cur2;
+            sh.queues = q;
+            step(sh);
+        }
+        // static form: tiles whose records exceed the window go to a second
+        // kernel with a larger window (same slots per workgroup)
+        cur2 = lt;
+        for (int h = 0; h < CACHE_TILE_COUNT && !cur2.persistent; ++h) {
+            if (CACHE_TILES[h].block != CACHE_TILES[cur2.tile].block ||
+                CACHE_TILES[h].window <= CACHE_TILES[cur2.tile].window)
+                continue;
+            vbt_batch_shape sh = cur2;
+            sh.heavyTile = h;
+            step(sh);
+        }
+        // with a heavy kernel, a smaller light window may now pay
+        cur2 = lt;
+        for (int t = 0; t < CACHE_TILE_COUNT && cur2.heavyTile >= 0; ++t) {
+            if (CACHE_TILES[t].block != CACHE_TILES[cur2.tile].block ||
+                CACHE_TILES[t].window >= CACHE_TILES[cur2.heavyTile].window)
+                continue;
+            vbt_batch_shape sh = cur2;
+            sh.tile = t;
+            step(sh);
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
