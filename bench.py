@@ -42,15 +42,16 @@ def algorithmic_bytes(info, nt: int) -> dict:
     return {"read": read, "written": written, "page_table": 4.0 * L, "total": read + written + 4.0 * L}
 
 
-def measured_traffic(workload: str):
-    """HBM bytes per launch of the frame-cache kernel for this workload, from
-    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (each scaled by what
-    the counter reads for a plain copy of known size), or None."""
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of this frame-cache kernel (tile shape) on this
+    workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (each
+    scaled by what the counter reads for a plain copy of known size), or
+    None when that kernel was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
-            rec = json.load(fh).get(workload)
-    except (OSError, ValueError):
+            rec = json.load(fh).get(workload, {}).get(kernel)
+    except (OSError, ValueError, AttributeError):
         return None
     return None if rec is None else rec.get("traffic_bytes")
 
@@ -445,6 +446,7 @@ def main():
         qres = query_bench(asset, hip, 65536, 50)
         small = query_bench(asset, hip, 256, 200)
 
+    kname = f"k_cache_render<0, {shapes[best][0]}, {shapes[best][1]}>"
     line = {
         "metric": "decoded GB/s + % HBM roofline; batched query p50/p99 us vs plain GPU decode",
         "value": round(decoded_gbs, 2),
@@ -466,9 +468,9 @@ def main():
                             "autotune_ms": dict(zip([f"{b}x{w}" for b, w in shapes], tile_ms))}},
         "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, kname),
                      "bytes_per_launch": byt["total"],
-                     "kernel": f"k_cache_render<0, {shapes[best][0]}, {shapes[best][1]}>",
+                     "kernel": kname,
                      "device_ceilings_gbs": ceil},
         "queries": {"random_65536": qres, "random_256": small},
     }

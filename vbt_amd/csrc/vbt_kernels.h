@@ -592,10 +592,12 @@ __device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t
     // flight) and decoded where the next step needs it
     uint32_t pOff = off + (k + 2u) * KB;
     RawKey raw = lds_fetch(L, pOff);
-    // step i's values for frame i's output, produced in step i + 1
+    // step i's values for frame i's output, produced in step i + 1; at the
+    // last key fkS is set past every frame, so the one comparison
+    // fkS >= frame selects vk both before the first key and after the last
+    // (a select, where `fk >= frame || k == last` compiled to two branches)
     int fkS = 0;
     float vkS = 0.0f, vnS = 0.0f;
-    bool endS = false;
     float gapS = 1.0f, yS = 1.0f;
     auto update = [&](int frame) {
         int fp;
@@ -614,17 +616,16 @@ __device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t
         y = adv ? yNew : y;
         pOff += step * KB;
         raw = lds_fetch(L, pOff);
-        fkS = fk;
+        fkS = k == last ? 0x7fffffff : fk;
         vkS = vk;
         vnS = vn;
-        endS = k == last;
         gapS = gap;
         yS = y;
     };
     auto output = [&](int frame) {
         const float alpha = sweep_alpha(frame - fkS, gapS, yS);
         const float lerp = vkS + alpha * (vnS - vkS);
-        const float val = (fkS >= frame || endS) ? vkS : lerp;
+        const float val = fkS >= frame ? vkS : lerp;
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(val), frame_rsrc(out, (uint32_t)strideBytes),
                                               (int)dstBytes, 0, 0);
         out += strideBytes;
