@@ -360,6 +360,8 @@ def main():
     ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
                     help="assets of the batched frame-cache build (all frames of each, one graph replay)")
     ap.add_argument("--tile", type=int, default=-1, help="frame-cache tile shape index (default: autotune)")
+    ap.add_argument("--single-frame-probe", type=int, default=0,
+                    help="also build this many one-frame packed caches (frames spread over the asset) for profiling")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -440,6 +442,16 @@ def main():
         print(f"[bench] copy probe: {nb} bytes read + {nb} bytes written by k_copy", file=sys.stderr, flush=True)
         src.free()
         dst.free()
+
+    if args.single_frame_probe:
+        one = hip.DeviceArray((1, info.controlsPerFrame), np.float32, s)
+        k = args.single_frame_probe
+        frames = [int(i * (F - 1) / max(k - 1, 1)) for i in range(k)]
+        for t in frames:
+            asset.frame_cache_into(one, t, 1)
+        s.sync()
+        print(f"[bench] single-frame probe: frames {frames}", file=sys.stderr, flush=True)
+        one.free()
 
     qres = small = None
     if not args.no_queries:
