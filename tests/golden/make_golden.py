@@ -49,6 +49,16 @@ FIXTURES = {
     "sci_mini": ("sci_mini", SCI_FLAGS, [0, 17, 47]),
     "sci_dense_mini": ("sci_dense_mini", SCI_FLAGS + ["--generic-norm-thr", "0.0005", "--generic-peak-thr", "0.0005"],
                        [0, 13, 39]),
+    # every leaf dense with the tile / patch temporal basis (dense subtypes 2
+    # and 3): the reference file decoder reads these blocks as per-frame BFP
+    # data (subtype 2 decodes, subtype 3 reads past the pool and throws)
+    "sci_tile_mini": ("sci_dense_mini", SCI_FLAGS + ["--generic-norm-thr", "0.0005", "--generic-peak-thr", "0.0005",
+                                                     "--generic-tile-temporal-basis", "--generic-tile-temporal-force"],
+                      [0, 13, 39]),
+    "sci_patch_mini": ("sci_dense_mini", SCI_FLAGS + ["--generic-norm-thr", "0.0005", "--generic-peak-thr", "0.0005",
+                                                      "--generic-patch-temporal-basis",
+                                                      "--generic-patch-temporal-force"],
+                       [0, 13, 39]),
 }
 
 
@@ -124,7 +134,14 @@ def sci_vectors(path: str, data: bytes, frames, nq=2000, seed=7):
     dense = []
     for t in frames:
         fr = np.empty((D, H, W), np.float32)
-        assert R.vbtref_sci_frame(h, t, fr.ctypes.data) == 0
+        if R.vbtref_sci_frame(h, t, fr.ctypes.data) != 0:
+            # the reference decoder throws at this frame: record its message
+            # and the frames that decoded before it
+            msg = R.vbtref_last_error().decode()
+            R.vbtref_free_file(h)
+            ok = frames[:len(dense)]
+            return {"frames": np.array(ok, np.int32), "dense": np.stack(dense) if dense else np.zeros((0, D, H, W),
+                    np.float32), "error": np.array(msg), "error_frame": np.array(t, np.int32)}
         dense.append(fr)
     out["dense"] = np.stack(dense)
     rng = np.random.default_rng(seed)
@@ -142,11 +159,20 @@ def sci_vectors(path: str, data: bytes, frames, nq=2000, seed=7):
 
 
 def main():
+    """python tests/golden/make_golden.py [NAME ...]: regenerate all fixtures,
+    or only the named ones (the manifest keeps the others)."""
     if not (os.path.exists(REF_KF) and os.path.exists(REF_SF) and oracle_py.have_ref()):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "-j4"], check=True)
+    only = sys.argv[1:]
     manifest = {}
+    mpath = os.path.join(HERE, "manifest.json")
+    if only and os.path.exists(mpath):
+        with open(mpath) as fh:
+            manifest = json.load(fh)
     with tempfile.TemporaryDirectory() as tmp:
         for name, (wname, extra, frames) in FIXTURES.items():
+            if only and name not in only:
+                continue
             w = synth.WORKLOADS[wname]
             data = encode_with_reference(w, extra, tmp)
             path = os.path.join(tmp, name + ".vbtp")
@@ -173,7 +199,7 @@ def main():
                 "modes": np.bincount(of.modes(), minlength=4).tolist(),
             }
             print(name, manifest[name])
-    with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+    with open(mpath, "w") as fh:
         json.dump(manifest, fh, indent=1)
 
 

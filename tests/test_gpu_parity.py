@@ -16,7 +16,7 @@ hip = pytest.importorskip("vbt_amd.hip")
 pytestmark = pytest.mark.gpu
 
 RENDER = ["smoke64", "liquid_mini"]
-SCI = ["sci_mini", "sci_dense_mini"]
+SCI = ["sci_mini", "sci_dense_mini", "sci_tile_mini"]
 ALL = RENDER + SCI
 
 
@@ -205,3 +205,31 @@ def test_fp32_tolerance_statement(assets):
     e = o.frame(17)
     rng = float(e.max() - e.min())
     assert np.max(np.abs(g - e)) <= 1e-6 * rng
+
+
+def test_dense_patch_subtype_reads_past_the_pool_like_the_reference():
+    """sci_patch_mini (dense subtype 3, read by the reference as per-frame
+    BFP data): the leaves whose reads run past the payload pool at some frame
+    are poisoned at open (for every frame: stricter than the reference, which
+    throws only at the frames that read past it) and whole-frame decodes
+    fail; point queries fail only in those leaves and elsewhere equal the
+    oracle."""
+    data = golden_io.vbtp("sci_patch_mini")
+    a = hip.Asset(data)
+    o = oracle_py.OracleFile(data)
+    assert a.info.badLeaves > 0
+    exp = golden_io.expected("sci_patch_mini")
+    with pytest.raises(hip.VbtError):
+        a.dense(int(exp["error_frame"]))
+    W, H, D, F = a.dims
+    rng = np.random.default_rng(4)
+    q = np.stack([rng.integers(0, W, 5000), rng.integers(0, H, 5000), rng.integers(0, D, 5000),
+                  rng.integers(0, F, 5000)], axis=1).astype(np.uint32)
+    vals, st = a.query(q, with_status=True)
+    exp = o.query(q, check=False)
+    good = st == 0
+    assert good.any() and (~good).any()
+    assert np.isnan(vals[~good]).all()
+    assert_bitexact(vals[good], exp[good], "patch-subtype queries in readable leaves")
+    a.close()
+

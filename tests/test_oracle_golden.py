@@ -9,7 +9,7 @@ import golden_io
 import oracle_py
 
 RENDER = ["smoke64", "liquid_mini"]
-SCI = ["sci_mini", "sci_dense_mini"]
+SCI = ["sci_mini", "sci_dense_mini", "sci_tile_mini"]
 
 
 @pytest.mark.parametrize("name", RENDER)
@@ -63,3 +63,26 @@ def test_golden_assets_cover_every_route():
     sci_modes = np.sum([m[n]["modes"] for n in SCI], axis=0)
     assert all(render_modes > 0), render_modes  # EMPTY, GRID4, Mode2 shell, FINE6
     assert sci_modes[1] > 0 and sci_modes[2] > 0 and sci_modes[3] > 0  # sparse, dense grid3, dense grid4
+
+
+def test_dense_subtypes_2_and_3_follow_the_reference_decoder():
+    """The reference file decoder has no reader for the tile / patch temporal
+    bases (dense subtypes 2 and 3): it reads them as per-frame BFP blocks.
+    For the tile fixture that decodes (matched bit for bit above); the patch
+    fixture's reads run past the payload pool, where the reference throws and
+    the oracle reports an out-of-range read."""
+    m = golden_io.manifest()
+    for name, sub in (("sci_tile_mini", 2), ("sci_patch_mini", 3)):
+        f = oracle_py.OracleFile(golden_io.vbtp(name))
+        words = f.payload.view(np.uint32)
+        hdr = words[f.offsets[:-1][f.offsets[1:] > f.offsets[:-1]]]
+        assert ((hdr & 3) >= 2).all() and (((hdr >> 12) & 7) == sub).all(), name
+        assert m[name]["modes"][2] == 48
+    exp = golden_io.expected("sci_patch_mini")
+    assert "range_check" in str(exp["error"])
+    f = oracle_py.OracleFile(golden_io.vbtp("sci_patch_mini"))
+    for i, t in enumerate(exp["frames"]):  # frames the reference decoded before throwing
+        np.testing.assert_array_equal(f.frame(int(t)).view(np.uint32), exp["dense"][i].view(np.uint32))
+    with pytest.raises(ValueError, match="out of range"):
+        f.frame(int(exp["error_frame"]))
+
