@@ -124,6 +124,10 @@ __device__ __forceinline__ void exact_tile(const AssetDev& a, uint32_t tile, int
     if (bad) atomicOr(err, 1u);
 }
 
+// staging depth of the batch kernels: their item loop keeps more state live
+// than the one-shot kernel, and deeper staging would cost them occupancy
+constexpr uint32_t BATCH_STAGE = 8u;
+
 template <uint32_t BLOCK, uint32_t WINDOW>
 __device__ __forceinline__ void batch_item(const BatchArgs& A, uint32_t it, uint32_t* lds, uint32_t* sRed)
 {
@@ -139,11 +143,11 @@ __device__ __forceinline__ void batch_item(const BatchArgs& A, uint32_t it, uint
         float* out = J.out + (uint64_t)(f0 - J.t0) * total;
         if (n <= CACHE_EXACT_MAX_FRAMES) exact_tile<BLOCK>(a, e.y, f0, n, out, total, J.err);
         else
-            cache_render_tile<VBT_CACHE_PACKED, BLOCK, WINDOW>(a, a.groupLeaf, e.y, f0, n, out, total, J.err, lds,
+            cache_render_tile<VBT_CACHE_PACKED, BLOCK, WINDOW, BATCH_STAGE>(a, a.groupLeaf, e.y, f0, n, out, total, J.err, lds,
                                                                sRed);
     } else {
         float* out = J.out + (uint64_t)(f0 - J.t0) * a.A * 576u;
-        cache_render_tile<VBT_CACHE_STRIDE576, BLOCK, WINDOW>(a, a.groupLeaf, e.y, f0, n, out, total, J.err, lds,
+        cache_render_tile<VBT_CACHE_STRIDE576, BLOCK, WINDOW, BATCH_STAGE>(a, a.groupLeaf, e.y, f0, n, out, total, J.err, lds,
                                                               sRed);
     }
 }

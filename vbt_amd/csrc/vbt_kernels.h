@@ -699,8 +699,11 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
 // slower: the key read sits on the loop's dependency chain.)
 // Tile: slots tile*BLOCK .. tile*BLOCK+BLOCK-1 of the packed cache; lds holds
 // WINDOW bytes, sRed BLOCK/64 words.  BLOCK and WINDOW are the tile shape (the
-// autotuner picks it for the one-shot kernel).
-template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
+// autotuner picks it for the one-shot kernel).  STAGE is the number of dword
+// loads each lane keeps in flight while staging a window: 32 brings a 32 KiB
+// window of a 256-slot tile in one round trip (the one-shot kernel stays at
+// 59 VGPRs); kernels with more live state use fewer to keep their occupancy.
+template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW, uint32_t STAGE = 32u>
 __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint32_t* groupLeaf, uint32_t tile, int t0,
                                                   int nt, float* out, uint64_t total, uint32_t* errFlag,
                                                   uint32_t* lds, uint32_t* sRed)
@@ -797,14 +800,17 @@ __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint3
         const bool mine = pending && ((waveFits ? waveHi <= limit : kHi <= limit) || kLo == lo);
         const uint32_t hi = block_max<NW>(mine ? kHi : 0u, sRed, lane, wave);
         const uint32_t nw = hi > w0 * 4u ? (hi + 3u) / 4u - w0 : 0u;
-        // staging: eight dword loads in flight per lane before the LDS writes
+        // staging: STAGE dword loads in flight per lane before the LDS writes
+        // (for the one-shot kernel 8 in flight measured 11 % slower per launch
+        // than 32; 48 spends registers)
+        constexpr uint32_t SU = STAGE;
         const uint32_t* pw = (const uint32_t*)a.payload + w0;
-        for (uint32_t i0 = tid; i0 < nw; i0 += 8u * BLOCK) {
-            uint32_t v[8];
+        for (uint32_t i0 = tid; i0 < nw; i0 += SU * BLOCK) {
+            uint32_t v[SU];
 #pragma unroll
-            for (uint32_t j = 0; j < 8u; ++j) v[j] = i0 + j * BLOCK < nw ? pw[i0 + j * BLOCK] : 0u;
+            for (uint32_t j = 0; j < SU; ++j) v[j] = i0 + j * BLOCK < nw ? pw[i0 + j * BLOCK] : 0u;
 #pragma unroll
-            for (uint32_t j = 0; j < 8u; ++j)
+            for (uint32_t j = 0; j < SU; ++j)
                 if (i0 + j * BLOCK < nw) lds[i0 + j * BLOCK] = v[j];
         }
         __syncthreads();
