@@ -340,6 +340,60 @@ def march_bench(hip, assets: dict, raws: dict, steps: int, cpu_threads: int, cpu
     return out
 
 
+def cycles_bench(hip, assets: dict, raws: dict, steps: int, cpu_threads: int, cpu_budget_s: float) -> dict:
+    """The Cycles resident blob and its samplers: the blob packed on the device
+    for one frame, then 2^20 uniform points inside the volume sampled by the
+    standalone form (8 lanes per sample) and the patched kernel's form (64
+    lanes per sample), each beside its one-lane-per-sample form; HIP events
+    around the launches; the oracle on the host cores for a bounded sample of
+    the same points."""
+    out = {}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    forms = {"standalone": 0, "standalone_plain": hip.CYCLES_PLAIN, "kernel": hip.CYCLES_KERNEL,
+             "kernel_plain": hip.CYCLES_KERNEL | hip.CYCLES_PLAIN}
+    for name, t in (("smoke256", 60), ("liquid320", 75)):
+        if name not in assets:
+            continue
+        a = assets[name]
+        W, H, D, F = a.dims
+        st = a.stream
+        blob = a.cycles_blob(t)
+        pack_ms = timed(hip, st, lambda: hip.check(hip.lib().vbt_cycles_pack_blob(a.h, t, blob.ptr, blob.nbytes, st.h)),
+                        max(3, steps // 4))
+        n = 1 << 20
+        pts = (np.random.default_rng(12).random((n, 3)) * [W - 1, H - 1, D - 1]).astype(np.float32)
+        d_pts = hip.DeviceArray.from_host(pts, st)
+        d_out = hip.DeviceArray(n, np.float32, st)
+        entry = {"points": n, "frame": t, "blob_mb": round(blob.nbytes / 1e6, 2), "pack_ms": round(pack_ms, 3)}
+        for form, flags in forms.items():
+            ms = timed(hip, st, lambda: hip.cycles_sample_into(blob, d_pts, n, d_out, flags, st), max(3, steps // 4))
+            entry[form] = {"ms": round(ms, 3), "samples_per_s": round(n / (ms * 1e-3), 1)}
+        entry["nonzero_fraction"] = round(float((d_out.host() != 0).mean()), 4)
+        if cpu_budget_s > 0 and name in raws:
+            import oracle_py
+            hb = blob.host()
+            for form, flags in (("standalone", 0), ("kernel", hip.CYCLES_KERNEL)):
+                m = 65536 if flags == 0 else 8192
+                sel = pts[:m]
+                t0 = time.perf_counter()
+                k = 0
+                while True:
+                    oracle_py.cycles_sample(hb, sel, flags, threads=cpu_threads)
+                    k += 1
+                    if time.perf_counter() - t0 > cpu_budget_s:
+                        break
+                dt = time.perf_counter() - t0
+                entry[form]["cpu_baseline"] = {"samples_per_s": round(k * m / dt, 1), "cores": cpu_threads,
+                                               "kind": "port", "sample": f"{k} x {m} of the same points in {dt:.1f} s"}
+        out[f"{name}_t{t}"] = entry
+        print(f"[bench] cycles {name}: blob {entry['blob_mb']} MB packed in {pack_ms:.3f} ms; "
+              + ", ".join(f"{f} {entry[f]['samples_per_s'] / 1e6:.1f} M/s" for f in forms)
+              + "".join(f"; CPU {f} {entry[f]['cpu_baseline']['samples_per_s'] / 1e6:.2f} M/s"
+                        for f in ("standalone", "kernel") if "cpu_baseline" in entry[f]),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,6 +410,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-ceilings", action="store_true")
     ap.add_argument("--no-march", action="store_true")
+    ap.add_argument("--no-cycles", action="store_true")
     ap.add_argument("--batch-tune-reps", type=int, default=2)
     ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
                     help="assets of the batched frame-cache build (all frames of each, one graph replay)")
@@ -490,7 +545,7 @@ def main():
     raws = {args.workload: data}
     want = ([] if args.no_batch else [x for x in args.batch_assets.split(",") if x]) + \
         ([] if args.no_serve else [x for x in args.serve_assets.split(",") if x]) + \
-        ([] if args.no_march else ["liquid320"])
+        ([] if args.no_march and args.no_cycles else ["liquid320"])
     for n in want:
         if n not in extra:
             t0 = time.time()
@@ -509,6 +564,9 @@ def main():
     if not args.no_march:
         line["march"] = march_bench(hip, extra, raws, args.steps, min(16, os.cpu_count() or 1),
                                     0.0 if args.no_cpu else 3.0)
+    if not args.no_cycles:
+        line["cycles"] = cycles_bench(hip, extra, raws, args.steps, min(16, os.cpu_count() or 1),
+                                      0.0 if args.no_cpu else 2.0)
     if not args.no_serve:
         serve = {n: extra[n] for n in [args.workload] + [x for x in args.serve_assets.split(",") if x]}
         line["serving"] = serving_bench(hip, loadgen, serve, args.serve_requests)

@@ -206,6 +206,18 @@ int vbto_sample_points(const vbto_file* f, const float* cache576, const vbto_mar
 int vbto_march_rays(const vbto_file* f, const float* cache576, const vbto_march_params* p, const float* rays,
                     uint64_t n, float* out, int nthreads);
 
+/* ---- Cycles resident blob and its samplers  (cycles_native_vbt/, blender_patch/native_cycles_vbt.patch) ---- */
+#define VBTO_CYCLES_KERNEL 1 /* the patched Cycles kernel's form; else the standalone header's */
+#define VBTO_CYCLES_WORLD 2  /* subtract the blob's bbox_min first (sample_density_world) */
+/* load_pack4's checks and pack_blob: writes the blob to out (when non-null and
+ * capacity allows) and its size to *blobBytes. */
+int vbto_cycles_pack_blob(const void* file, uint64_t fileBytes, int frameIndex, void* out, uint64_t capacity,
+                          uint64_t* blobBytes);
+/* density at n points xyz (3 floats each) read from a blob */
+int vbto_cycles_sample(const void* blob, uint64_t blobBytes, int flags, const float* xyz, uint64_t n, float* out,
+                       int nthreads);
+const char* vbto_cycles_error(void);
+
 /* ---- scientific route  (render/src/vbt_file.cpp:220-248, render/src/scientific_decode.cpp) ---- */
 typedef struct {
     uint32_t mode, coarseKeepMinus1, coarseQuantClass, tierId, framesPerBinMinus1, eventQuantCode,

@@ -116,6 +116,11 @@ def oracle():
         L.vbto_sci_leaf_frame.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
         L.vbto_decode_render_leaf_frame.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32, ctypes.c_int,
                                                     ctypes.POINTER(Compose), ctypes.c_void_p]
+        L.vbto_cycles_pack_blob.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+        L.vbto_cycles_sample.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int]
+        L.vbto_cycles_error.restype = ctypes.c_char_p
         _O = L
     return _O
 
@@ -265,3 +270,31 @@ class OracleFile:
 
     def modes(self) -> np.ndarray:
         return (self.payload.view(np.uint32)[self.offsets[:-1]] & 3).astype(np.uint8)
+
+
+CYCLES_KERNEL, CYCLES_WORLD = 1, 2
+
+
+def cycles_blob(data: bytes, frame: int) -> np.ndarray:
+    """The Cycles resident blob of a VBTPACK4 image (load_pack4 + pack_blob)."""
+    L = oracle()
+    buf = np.frombuffer(data, np.uint8)
+    n = ctypes.c_uint64()
+    if L.vbto_cycles_pack_blob(buf.ctypes.data, len(data), frame, None, 0, ctypes.byref(n)) != 0:
+        raise ValueError(L.vbto_cycles_error().decode())
+    out = np.zeros(n.value, np.uint8)
+    if L.vbto_cycles_pack_blob(buf.ctypes.data, len(data), frame, out.ctypes.data, n.value, ctypes.byref(n)) != 0:
+        raise ValueError(L.vbto_cycles_error().decode())
+    return out
+
+
+def cycles_sample(blob: np.ndarray, xyz: np.ndarray, flags: int = 0, threads: int = 1) -> np.ndarray:
+    """Density at n points read from a blob (standalone or kernel form)."""
+    blob = np.ascontiguousarray(blob, np.uint8)
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    out = np.empty(len(xyz), np.float32)
+    L = oracle()
+    if L.vbto_cycles_sample(blob.ctypes.data, len(blob), flags, xyz.ctypes.data, len(xyz), out.ctypes.data,
+                            threads) != 0:
+        raise ValueError(L.vbto_cycles_error().decode())
+    return out

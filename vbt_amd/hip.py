@@ -158,6 +158,10 @@ def lib():
                                         c_void_p, c_void_p]
         L.vbt_march_rays.argtypes = [c_void_p, c_void_p, ctypes.POINTER(MarchParams), c_void_p, c_u32, c_void_p,
                                      c_void_p]
+        L.vbt_cycles_blob_bytes.argtypes = [c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+        L.vbt_cycles_pack_blob.argtypes = [c_void_p, c_int, c_void_p, ctypes.c_uint64, c_void_p]
+        L.vbt_cycles_sample_density.argtypes = [c_void_p, ctypes.c_uint64, c_int, c_void_p, c_u32, c_void_p,
+                                                c_void_p]
         L.vbt_decode_control_values.argtypes = [c_void_p, c_u32, ctypes.c_uint16, ctypes.c_uint8, ctypes.c_uint8,
                                                 c_u32, c_u32, c_int, c_int, c_void_p, c_void_p, c_u32, c_void_p,
                                                 c_void_p, c_void_p]
@@ -290,6 +294,28 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+CYCLES_KERNEL, CYCLES_WORLD, CYCLES_PLAIN = 1, 2, 4
+
+
+def cycles_sample_into(blob: DeviceArray, xyz: DeviceArray, n: int, out: DeviceArray, flags: int = 0,
+                       stream: Stream | None = None):
+    check(lib().vbt_cycles_sample_density(blob.ptr, blob.nbytes, int(flags), xyz.ptr, n, out.ptr,
+                                          (stream or blob.stream).h if (stream or blob.stream) else None))
+
+
+def cycles_sample(blob: DeviceArray, xyz: np.ndarray, flags: int = 0) -> np.ndarray:
+    """Density at n points (n x 3) read from a device Cycles blob:
+    the standalone sampler, or with CYCLES_KERNEL the patched Cycles kernel's."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+    n = len(xyz)
+    if n == 0:
+        return np.zeros(0, np.float32)
+    d = DeviceArray.from_host(xyz, blob.stream)
+    out = DeviceArray(n, np.float32, blob.stream)
+    cycles_sample_into(blob, d, n, out, flags)
+    return out.host()
 
 
 class Event:
@@ -454,6 +480,15 @@ class Asset:
                    stream: Stream | None = None):
         check(lib().vbt_march_rays(self.h, cache576.ptr, ctypes.byref(params), rays.ptr, n, out.ptr,
                                    (stream or self.stream).h))
+
+    def cycles_blob(self, frame_index: int) -> DeviceArray:
+        """The Cycles resident blob (header, offsets, payload) for one frame,
+        packed on the device (cycles_native_vbt/vbt_cycles_blob.cpp:22-92)."""
+        n = ctypes.c_uint64()
+        check(lib().vbt_cycles_blob_bytes(self.h, ctypes.byref(n)))
+        blob = DeviceArray(n.value, np.uint8, self.stream)
+        check(lib().vbt_cycles_pack_blob(self.h, int(frame_index), blob.ptr, n.value, self.stream.h))
+        return blob
 
     def march(self, cache576: DeviceArray, params: MarchParams, rays: np.ndarray) -> np.ndarray:
         """The viewer's ray march (n x 8 rays: origin, tmin, direction, tmax)
