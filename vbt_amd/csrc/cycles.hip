@@ -118,7 +118,7 @@ __device__ __forceinline__ float key_value(const BlobView& v, uint32_t keys, uin
 
 // decode_temporal_control_value (vbt_cycles_sampler.h:163-213; the patch's
 // copy, native_cycles_vbt.patch:639-685, is the same)
-__device__ float control_value(const BlobView& v, uint32_t stream, uint32_t codec, uint32_t nc, uint32_t nkeys,
+__device__ __forceinline__ float control_value(const BlobView& v, uint32_t stream, uint32_t codec, uint32_t nc, uint32_t nkeys,
                                uint32_t ci)
 {
     const uint32_t descBytes = nc * 4u;
@@ -215,7 +215,7 @@ __device__ __forceinline__ float serial_mix(const float* c, float wx, float wy, 
 // sample_control_stream (vbt_cycles_sampler.h:248-283).  G = 8: lane c of an
 // aligned group of 8 decodes lattice corner c; G = 1: one lane decodes all 8.
 template <bool KF, uint32_t G>
-__device__ float stream_sample(const BlobView& v, uint32_t stream, uint32_t codec, uint32_t res, uint32_t nkeys,
+__device__ __forceinline__ float stream_sample(const BlobView& v, uint32_t stream, uint32_t codec, uint32_t res, uint32_t nkeys,
                                uint32_t ex, uint32_t ey, uint32_t ez, float lx, float ly, float lz, uint32_t c)
 {
     if (res == 0u) return 0.0f;
@@ -244,7 +244,7 @@ __device__ float stream_sample(const BlobView& v, uint32_t stream, uint32_t code
 // form's vbt_sample_density_leaf_index (native_cycles_vbt.patch:788-874):
 // Mode2 leaves give 0 and leaf_count_z comes from the header.
 template <bool KF, uint32_t G>
-__device__ float leaf_sample(const BlobView& v, float x, float y, float z, uint32_t c)
+__device__ __forceinline__ float leaf_sample(const BlobView& v, float x, float y, float z, uint32_t c)
 {
     if (x < 0.0f || y < 0.0f || z < 0.0f || x > (float)(v.dimX - 1u) || y > (float)(v.dimY - 1u) ||
         z > (float)(v.dimZ - 1u))
