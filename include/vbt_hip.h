@@ -479,21 +479,21 @@ int vbt_march_rays(vbt_asset a, const float* d_cache576, const vbt_march_params*
  * leaf offset table and the payload pool, the layout the patched Cycles
  * kernel reads (native_cycles_vbt.patch:456-476, 551-563). */
 #define VBT_CYCLES_KERNEL 1 /* the patched Cycles kernel's sampler (native_cycles_vbt.patch:876-929);
-                               else the standalone header's (vbt_cycles_sampler.h:285-376) */
-#define VBT_CYCLES_WORLD 2  /* subtract the blob's bbox_min first (vbt_cycles_sampler.h:369-375) */
+                               else the standalone header's (vbt_cycles_sampler.h:284-376) */
+#define VBT_CYCLES_WORLD 2  /* subtract the blob's bbox_min first (vbt_cycles_sampler.h:370-376) */
 #define VBT_CYCLES_PLAIN 4  /* one lane per sample, the reference's structure (for comparison) */
 
-/* Size of the blob pack_blob makes for this asset (vbt_cycles_blob.cpp:52-55). */
+/* Size of the blob pack_blob makes for this asset (vbt_cycles_blob.cpp:50-52). */
 int vbt_cycles_blob_bytes(vbt_asset a, uint64_t* bytes);
-/* pack_blob (vbt_cycles_blob.cpp:22-92) on the device from the asset's resident
+/* pack_blob (vbt_cycles_blob.cpp:22-87) on the device from the asset's resident
  * offsets and payload: frameIndex is clamped to [0, frames - 1]; d_blob is
  * 4-byte aligned with room for vbt_cycles_blob_bytes.  Stream-ordered. */
 int vbt_cycles_pack_blob(vbt_asset a, int frameIndex, void* d_blob, uint64_t capacity, void* stream);
 /* Density at n points (d_xyz: 3 floats each, index space unless
  * VBT_CYCLES_WORLD) read from a device blob: sample_density_index
- * (vbt_cycles_sampler.h:285-367) or, with VBT_CYCLES_KERNEL,
+ * (vbt_cycles_sampler.h:284-368) or, with VBT_CYCLES_KERNEL,
  * kernel_image_interp_vbt_density (native_cycles_vbt.patch:915-929).  A blob
- * whose header fails blob_payload_view's checks (vbt_cycles_blob.cpp:95-113)
+ * whose header fails blob_payload_view's checks (vbt_cycles_blob.cpp:89-124)
  * samples to 0, as the Cycles kernel does for a bad magic.  Reads past the
  * blob's sections read zero bytes. */
 int vbt_cycles_sample_density(const void* d_blob, uint64_t blobBytes, int flags, const float* d_xyz, uint32_t n,

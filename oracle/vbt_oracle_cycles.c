@@ -69,7 +69,7 @@ static uint32_t rd32(const uint8_t* p)
 static void wr32(uint8_t* p, uint32_t v) { memcpy(p, &v, 4); }
 
 /* load_pack4 (vbt_cycles_loader.cpp:116-221) with validate_header (:36-114),
- * then pack_blob (vbt_cycles_blob.cpp:22-92). */
+ * then pack_blob (vbt_cycles_blob.cpp:22-87). */
 int vbto_cycles_pack_blob(const void* file, uint64_t fileBytes, int frameIndex, void* out, uint64_t capacity,
                           uint64_t* blobBytes)
 {
@@ -185,7 +185,7 @@ static float key_value_at(const view_t* v, uint32_t keys, uint32_t codec, uint32
 
 static float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
-/* decode_temporal_control_value, vbt_cycles_sampler.h:163-213 (the patch's
+/* decode_temporal_control_value, vbt_cycles_sampler.h:165-212 (the patch's
  * copy at native_cycles_vbt.patch:639-685 is the same): the descriptor's key
  * range, a bracket from the frame's bin hint and the next bin's, a binary
  * search inside it, and the lerp weight as an IEEE quotient of the unsigned
@@ -243,7 +243,7 @@ static float ctrl_pos(uint32_t i, uint32_t res)
     return res <= 1u ? 0.0f : (float)i * (7.0f / (float)(res - 1u));
 }
 
-/* locate_coord, vbt_cycles_sampler.h:221-241; the kernel form
+/* locate_coord, vbt_cycles_sampler.h:219-243; the kernel form
  * (native_cycles_vbt.patch:694-719) first sends resolution <= 1 to (0, 0, 0). */
 static void locate(float p, uint32_t extent, uint32_t res, int kernelForm, uint32_t* i0, uint32_t* i1, float* w)
 {
@@ -263,7 +263,7 @@ static void locate(float p, uint32_t extent, uint32_t res, int kernelForm, uint3
     *w = c1 > c0 ? (c - c0) / (c1 - c0) : 0.0f;
 }
 
-/* sample_control_stream, vbt_cycles_sampler.h:248-283 */
+/* sample_control_stream, vbt_cycles_sampler.h:245-282 */
 static float stream_sample(const view_t* v, uint32_t stream, uint32_t codec, uint32_t res, uint32_t nkeys,
                            const uint32_t ext[3], const float l[3], int kernelForm)
 {
@@ -282,7 +282,7 @@ static float stream_sample(const view_t* v, uint32_t stream, uint32_t codec, uin
     return mixf(mixf(c00, c10, w[1]), mixf(c01, c11, w[1]), w[2]);
 }
 
-/* sample_density_index, vbt_cycles_sampler.h:285-367; the kernel form's
+/* sample_density_index, vbt_cycles_sampler.h:284-368; the kernel form's
  * vbt_sample_density_leaf_index (native_cycles_vbt.patch:788-874) differs in
  * Mode2 leaves (zero), leaf_count_z (from the header) and resolution <= 1. */
 static float leaf_sample(const view_t* v, float x, float y, float z, int kernelForm)
@@ -349,7 +349,7 @@ static float voxel_trilinear(const view_t* v, float x, float y, float z)
     return mixf(mixf(c00, c10, t[1]), mixf(c01, c11, t[1]), t[2]);
 }
 
-/* blob_payload_view's checks (vbt_cycles_blob.cpp:95-124) plus the kernel's
+/* blob_payload_view's checks (vbt_cycles_blob.cpp:89-124) plus the kernel's
  * magic test; a blob failing them samples to 0 everywhere in the kernel form
  * and is an error in the standalone form. */
 static int open_view(const uint8_t* blob, uint64_t bytes, view_t* v)

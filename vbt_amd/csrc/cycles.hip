@@ -57,7 +57,7 @@ struct BlobView {
     float bx, by, bz;
 };
 
-// blob_payload_view's checks (vbt_cycles_blob.cpp:95-113) and the kernel's
+// blob_payload_view's checks (vbt_cycles_blob.cpp:89-124) and the kernel's
 // magic test; sections must also sit on 4-byte boundaries.
 __device__ __forceinline__ bool open_view(const uint32_t* blob, uint64_t bytes, BlobView& v)
 {
@@ -116,7 +116,7 @@ __device__ __forceinline__ float key_value(const BlobView& v, uint32_t keys, uin
     return half_to_float((bytes4(v, keys + 3u * k) >> 8) & 0xFFFFu);
 }
 
-// decode_temporal_control_value (vbt_cycles_sampler.h:163-213; the patch's
+// decode_temporal_control_value (vbt_cycles_sampler.h:165-212; the patch's
 // copy, native_cycles_vbt.patch:639-685, is the same)
 __device__ __forceinline__ float control_value(const BlobView& v, uint32_t stream, uint32_t codec, uint32_t nc, uint32_t nkeys,
                                uint32_t ci)
@@ -169,7 +169,7 @@ __device__ __forceinline__ float ctrl_pos(uint32_t i, uint32_t res)
     return res <= 1u ? 0.0f : (float)i * (7.0f / (float)(res - 1u));
 }
 
-// locate_coord (vbt_cycles_sampler.h:221-241); the kernel form
+// locate_coord (vbt_cycles_sampler.h:219-243); the kernel form
 // (native_cycles_vbt.patch:694-719) sends resolution <= 1 to (0, 0, 0)
 template <bool KF>
 __device__ __forceinline__ void locate(float p, uint32_t extent, uint32_t res, uint32_t& i0, uint32_t& i1, float& w)
@@ -212,7 +212,7 @@ __device__ __forceinline__ float serial_mix(const float* c, float wx, float wy, 
     return mixf(mixf(c00, c10, wy), mixf(c01, c11, wy), wz);
 }
 
-// sample_control_stream (vbt_cycles_sampler.h:248-283).  G = 8: lane c of an
+// sample_control_stream (vbt_cycles_sampler.h:245-282).  G = 8: lane c of an
 // aligned group of 8 decodes lattice corner c; G = 1: one lane decodes all 8.
 template <bool KF, uint32_t G>
 __device__ __forceinline__ float stream_sample(const BlobView& v, uint32_t stream, uint32_t codec, uint32_t res, uint32_t nkeys,
@@ -240,7 +240,7 @@ __device__ __forceinline__ float stream_sample(const BlobView& v, uint32_t strea
     }
 }
 
-// sample_density_index (vbt_cycles_sampler.h:285-367); with KF the kernel
+// sample_density_index (vbt_cycles_sampler.h:284-368); with KF the kernel
 // form's vbt_sample_density_leaf_index (native_cycles_vbt.patch:788-874):
 // Mode2 leaves give 0 and leaf_count_z comes from the header.
 template <bool KF, uint32_t G>

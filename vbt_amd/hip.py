@@ -483,7 +483,7 @@ class Asset:
 
     def cycles_blob(self, frame_index: int) -> DeviceArray:
         """The Cycles resident blob (header, offsets, payload) for one frame,
-        packed on the device (cycles_native_vbt/vbt_cycles_blob.cpp:22-92)."""
+        packed on the device (cycles_native_vbt/vbt_cycles_blob.cpp:22-87)."""
         n = ctypes.c_uint64()
         check(lib().vbt_cycles_blob_bytes(self.h, ctypes.byref(n)))
         blob = DeviceArray(n.value, np.uint8, self.stream)
