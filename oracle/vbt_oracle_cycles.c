@@ -3,7 +3,7 @@
  * VBTPACK4: the resident blob (cycles_native_vbt/vbt_cycles_blob.{h,cpp}, the
  * file checks of cycles_native_vbt/vbt_cycles_loader.cpp) and the two density
  * samplers that read it straight from the packed payload:
- *   standalone  cycles_native_vbt/vbt_cycles_sampler.h:219-376, the form the
+ *   standalone  cycles_native_vbt/vbt_cycles_sampler.h:165-376, the form the
  *               reference's CPU probe (vbt_sampler_cpu_probe.cpp) runs;
  *   kernel      blender_patch/native_cycles_vbt.patch:688-929, the form the
  *               patched Cycles kernel runs on the device (voxel-corner
@@ -31,7 +31,7 @@
  *                        golden render assets (density, and the level set with
  *                        its Mode2 leaves), at random, voxel, leaf-boundary and
  *                        outside points and at several frames, also through
- *                        the committed fixture tests/golden/cycles_probe.json;
+ *                        the committed fixture tests/golden/cycles_probe.npz;
  *   blob                 size equal to the probe's reported blob size; the
  *                        header fields equal the file header's, and the
  *                        offset and payload sections equal the file's bytes;

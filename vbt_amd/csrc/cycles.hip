@@ -3,7 +3,7 @@
 // header, the leaf offset table, the payload pool) built from an open asset
 // without touching the host, and the two density samplers that read that
 // blob straight from the packed payload:
-//   standalone  cycles_native_vbt/vbt_cycles_sampler.h:219-376, the form the
+//   standalone  cycles_native_vbt/vbt_cycles_sampler.h:165-376, the form the
 //               reference's CPU probe runs: trilinear over each stream's
 //               control lattice at the point's position inside its leaf;
 //   kernel      blender_patch/native_cycles_vbt.patch:688-929, the form the
