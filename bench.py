@@ -340,6 +340,33 @@ def march_bench(hip, assets: dict, raws: dict, steps: int, cpu_threads: int, cpu
     return out
 
 
+def single_frame_bench(hip, asset, info, F: int, s) -> dict:
+    """One-frame packed-cache builds at 24 frames spread over the sequence,
+    HIP events around each build, with the bracket search from the payload
+    and then with the frame index (its size and build time reported)."""
+    one = hip.DeviceArray((1, info.controlsPerFrame), np.float32, s)
+    frames = [int(i * (F - 1) / 23) for i in range(24)]
+
+    def run():
+        for t in frames[:4]:
+            asset.frame_cache_into(one, t, 1)
+        return timed(hip, s, lambda: [asset.frame_cache_into(one, t, 1) for t in frames], 3, 0) / len(frames)
+
+    exact_ms = run()
+    t0 = time.perf_counter()
+    nbytes = asset.build_frame_index()
+    build_s = time.perf_counter() - t0
+    index_ms = run()
+    asset.drop_frame_index()
+    one.free()
+    out = {"frames": len(frames), "exact_us": round(exact_ms * 1e3, 2), "indexed_us": round(index_ms * 1e3, 2),
+           "index_mb": round(nbytes / 1e6, 1), "index_build_ms": round(build_s * 1e3, 1),
+           "out_mb_per_frame": round(4 * info.controlsPerFrame / 1e6, 2)}
+    print(f"[bench] one-frame builds: {out['exact_us']} us (bracket search) vs {out['indexed_us']} us (frame index, "
+          f"{out['index_mb']} MB, built in {out['index_build_ms']} ms)", file=sys.stderr, flush=True)
+    return out
+
+
 def cycles_bench(hip, assets: dict, raws: dict, steps: int, cpu_threads: int, cpu_budget_s: float) -> dict:
     """The Cycles resident blob and its samplers: the blob packed on the device
     for one frame, then 2^20 uniform points inside the volume sampled by the
@@ -415,6 +442,8 @@ def main():
     ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
                     help="assets of the batched frame-cache build (all frames of each, one graph replay)")
     ap.add_argument("--tile", type=int, default=-1, help="frame-cache tile shape index (default: autotune)")
+    ap.add_argument("--single-frame-index", action="store_true",
+                    help="build the frame index before the single-frame probe")
     ap.add_argument("--single-frame-probe", type=int, default=0,
                     help="also build this many one-frame packed caches (frames spread over the asset) for profiling")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -498,7 +527,10 @@ def main():
         src.free()
         dst.free()
 
+    single = single_frame_bench(hip, asset, info, F, s)
     if args.single_frame_probe:
+        if args.single_frame_index:
+            asset.build_frame_index()
         one = hip.DeviceArray((1, info.controlsPerFrame), np.float32, s)
         k = args.single_frame_probe
         frames = [int(i * (F - 1) / max(k - 1, 1)) for i in range(k)]
@@ -507,6 +539,7 @@ def main():
         s.sync()
         print(f"[bench] single-frame probe: frames {frames}", file=sys.stderr, flush=True)
         one.free()
+        asset.drop_frame_index()
 
     qres = small = None
     if not args.no_queries:
@@ -540,6 +573,7 @@ def main():
                      "kernel": kname,
                      "device_ceilings_gbs": ceil},
         "queries": {"random_65536": qres, "random_256": small},
+        "single_frame": single,
     }
     extra = {args.workload: asset}
     raws = {args.workload: data}

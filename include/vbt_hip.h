@@ -473,6 +473,19 @@ int vbt_sample_points(vbt_asset a, const float* d_cache576, const vbt_march_para
 int vbt_march_rays(vbt_asset a, const float* d_cache576, const vbt_march_params* p, const float* d_rays, uint32_t n,
                    float* d_out, void* stream);
 
+/* Frame index for repeated one- and two-frame packed-cache builds (serving,
+ * scrubbing): each slot's keyframe records regrouped by eighth of the frame
+ * range, so such a build reads short adjacent runs instead of one keyframe
+ * line per slot.  No counterpart in the reference (its decoder reads the
+ * payload per value); it changes no result: with the index, frame builds of
+ * one or two frames in the packed layout read it, other builds and leaves
+ * whose keys do not strictly increase keep the bracket search.  Costs
+ * 4 * 8 * (controlsPerFrame + 1) bytes of offsets plus 8 bytes per indexed
+ * record (*bytes); synchronizes the stream.  Render assets only. */
+int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream);
+/* Frees the frame index (waits for the device first). */
+int vbt_asset_drop_frame_index(vbt_asset a);
+
 /* ---- Cycles resident blob (cycles_native_vbt/, blender_patch/native_cycles_vbt.patch) ----
  * The blob is the reference's BlobHeader (cycles_native_vbt/vbt_cycles_blob.h:14-34:
  * 19 four-byte fields, magic "VBT1", version 1, bbox_min 0) followed by the

@@ -25,6 +25,7 @@
 #include "vbt_kernels.h"
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstdio>
@@ -390,6 +391,150 @@ __global__ void __launch_bounds__(256) k_cache_exact(AssetDev a, int t0, int nt,
     for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = decode_control_w(W, s, ci, (int)a.F, t0 + i);
 }
 
+// ------------------------------------------- frame index for few-frame builds --
+// A copy of each slot's keyframe records regrouped by time bin: bin b holds,
+// for every packed-cache slot in order, the slot's records from the last key
+// at or before the bin's first frame to the first key at or after its last
+// frame, as {frame, value bits}.  A one- or two-frame build then reads each
+// slot's bracket from a short run that sits next to its neighbours' runs,
+// instead of one line of the keyframe pool per slot.  Leaves whose keys do not
+// strictly increase, poisoned leaves and empty streams keep no run and take
+// the reference bracket search (cache_slot_exact).
+constexpr uint32_t INDEX_BINS = 8u;
+
+__device__ __forceinline__ uint32_t index_bin(int t, uint32_t F) { return min(INDEX_BINS - 1u, (uint32_t)t * INDEX_BINS / F); }
+
+// the run [lo, hi] of records of one slot for bin b; false when the slot keeps none
+__device__ __forceinline__ bool index_run(const AssetDev& a, uint64_t g, uint32_t b, const uint32_t*& W,
+                                          StreamRef& s, uint32_t& k0, uint32_t& lo, uint32_t& hi)
+{
+    uint32_t c = a.groupLeaf[g / CACHE_GROUP];
+    while (c + 1u < a.A && a.ctrl[c + 1u] <= g) ++c;
+    const DevLeaf& d = a.leaves[c];
+    if ((d.status & STATUS_MASK) != LEAF_OK || !(d.status & FLAG_SWEEP)) return false;
+    const uint32_t F = a.F;
+    const uint32_t t0 = (b * F + INDEX_BINS - 1u) / INDEX_BINS;
+    const uint32_t t1 = b + 1u < INDEX_BINS ? ((b + 1u) * F + INDEX_BINS - 1u) / INDEX_BINS - 1u : F - 1u;
+    if (t0 > t1) return false;
+    const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
+    const bool isCoarse = slot < d.cc;
+    s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
+    const uint32_t ci = isCoarse ? slot : slot - d.cc;
+    W = (const uint32_t*)(a.payload + (size_t)d.word * 4u);
+    const uint32_t desc = W[(s.base + ci * 4u) >> 2];
+    const uint32_t keyStart = desc & 0xFFFFu, keyCount = (desc >> 16) & 0xFFu;
+    if (keyCount == 0u) return false;
+    const uint32_t kb = keyframe_bytes(s.codec);
+    k0 = s.keyBase + keyStart * kb;
+    // n0 = keys with frame <= t0, n1 = keys with frame < t1 (frames increase)
+    uint32_t l = 0u, h = keyCount;
+    while (l < h) {
+        const uint32_t m = (l + h) / 2u;
+        int f;
+        float v;
+        w_rec(W, k0 + m * kb, s.codec, f, v);
+        if ((uint32_t)f <= t0) l = m + 1u;
+        else h = m;
+    }
+    const uint32_t n0 = l;
+    l = 0u;
+    h = keyCount;
+    while (l < h) {
+        const uint32_t m = (l + h) / 2u;
+        int f;
+        float v;
+        w_rec(W, k0 + m * kb, s.codec, f, v);
+        if ((uint32_t)f < t1) l = m + 1u;
+        else h = m;
+    }
+    lo = n0 > 0u ? n0 - 1u : 0u;
+    hi = l < keyCount ? l : keyCount - 1u;
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_index_count(AssetDev a, uint32_t* cnt, uint64_t C)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (g > C) return;
+    for (uint32_t b = 0; b < INDEX_BINS; ++b) {
+        const uint32_t*W;
+        StreamRef s;
+        uint32_t k0, lo, hi;
+        cnt[b * (C + 1u) + g] = g < C && index_run(a, g, b, W, s, k0, lo, hi) ? hi - lo + 1u : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_index_scatter(AssetDev a, const uint32_t* off, uint2* rec, uint64_t C)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (g >= C) return;
+    for (uint32_t b = 0; b < INDEX_BINS; ++b) {
+        const uint32_t* W;
+        StreamRef s;
+        uint32_t k0, lo, hi;
+        if (!index_run(a, g, b, W, s, k0, lo, hi)) continue;
+        uint2* o = rec + off[b * (C + 1u) + g];
+        const uint32_t kb = keyframe_bytes(s.codec);
+        for (uint32_t k = lo; k <= hi; ++k) {
+            int f;
+            float v;
+            w_rec(W, k0 + k * kb, s.codec, f, v);
+            o[k - lo] = make_uint2((uint32_t)f, __float_as_uint(v));
+        }
+    }
+}
+
+// the sum of n counts, so the host can check it fits 32-bit record offsets
+__global__ void __launch_bounds__(256) k_sum64(const uint32_t* cnt, uint64_t n, unsigned long long* sum)
+{
+    unsigned long long acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) acc += cnt[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63u) == 0u && acc) atomicAdd(sum, acc);
+}
+
+// Packed cache, nt <= 2 frames, from the frame index: the same values as
+// k_cache_exact (for strictly increasing keys the reference bracket is the
+// last key at or before the frame and the next one).
+__global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_t* off, const uint2* rec, int t0,
+                                                       int nt, float* out, uint64_t total, uint32_t* errFlag)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (g >= total) return;
+    for (int i = 0; i < nt; ++i) {
+        const int t = t0 + i;
+        const uint32_t b = index_bin(t, a.F);
+        const uint32_t o0 = off[b * (total + 1u) + g], o1 = off[b * (total + 1u) + g + 1u];
+        float val;
+        if (o0 == o1) {
+            int32_t st;
+            val = cache_slot_exact(a, g, t, st);
+            if (st) atomicOr(errFlag, 1u);
+        } else {
+            uint32_t j = o0;
+            uint2 r = rec[j];
+            if (t < (int)r.x) {
+                val = __uint_as_float(r.y);
+            } else {
+                uint2 n = j + 1u < o1 ? rec[j + 1u] : r;
+                while (j + 1u < o1 && (int)n.x <= t) {
+                    ++j;
+                    r = n;
+                    if (j + 1u < o1) n = rec[j + 1u];
+                }
+                if ((int)r.x == t || j + 1u == o1) {
+                    val = __uint_as_float(r.y);
+                } else {
+                    const float v0 = __uint_as_float(r.y), v1 = __uint_as_float(n.y);
+                    const float alpha = (float)(t - (int)r.x) / (float)((int)n.x - (int)r.x);
+                    val = v0 + alpha * (v1 - v0);
+                }
+            }
+        }
+        out[(uint64_t)i * total + g] = val;
+    }
+}
+
 template <int LAYOUT>
 void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* groupLeaf, int t0, int nt, float* out,
                          uint64_t total, uint32_t* err, hipStream_t s)
@@ -704,6 +849,9 @@ struct vbt_asset_s {
     uint32_t* d_blockLeaf = nullptr; // first active leaf of each 64-slot group of the packed cache
     int cacheTile = CACHE_TILE_DEFAULT;
     Pair* d_blocks = nullptr;
+    uint32_t* d_binOff = nullptr; // frame index: record offsets, INDEX_BINS x (controlsPerFrame + 1)
+    uint2* d_binRec = nullptr;    // frame index: {frame, value bits} records
+    uint64_t binRecords = 0;
     uint32_t A = 0, bad = 0;
     uint64_t controlsPerFrame = 0, payloadBytes = 0, deviceBytes = 0, fast = 0;
     uint64_t modeCounts[4] = {0, 0, 0, 0};
@@ -748,7 +896,8 @@ void free_asset(vbt_asset a)
 {
     if (!a) return;
     void* ptrs[] = {a->d_offsets, a->d_payload, a->d_scales, a->d_compact, a->d_status,     a->d_leaves,
-                    a->d_ctrl,    a->d_basis,   a->d_err,    a->d_blocks,  a->d_blockLeaf};
+                    a->d_ctrl,    a->d_basis,   a->d_err,    a->d_blocks,  a->d_blockLeaf, a->d_binOff,
+                    a->d_binRec};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete a;
@@ -965,7 +1114,10 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     if (a->h.profileType == 2) {
         k_sci_leaf<false><<<dim3(a->A, (uint32_t)nt), 512, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
     } else if (layout == VBT_CACHE_PACKED) {
-        if (nt <= CACHE_EXACT_MAX_FRAMES)
+        if (nt <= CACHE_EXACT_MAX_FRAMES && a->d_binOff)
+            k_cache_indexed<<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
+                ad, a->d_binOff, a->d_binRec, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        else if (nt <= CACHE_EXACT_MAX_FRAMES)
             k_cache_exact<VBT_CACHE_PACKED><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
                 ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
         else
@@ -1071,6 +1223,100 @@ int vbt_cache_tile_shape(int tile, uint32_t* block, uint32_t* window)
     if (tile < 0 || tile >= CACHE_TILE_COUNT) return vbt_fail(VBT_ERR_INVALID, "tile index out of range");
     *block = CACHE_TILES[tile].block;
     *window = CACHE_TILES[tile].window;
+    return VBT_OK;
+}
+
+int vbt_asset_drop_frame_index(vbt_asset a)
+{
+    if (!a) return vbt_fail(VBT_ERR_INVALID, "vbt_asset_drop_frame_index: null asset");
+    if (a->d_binOff || a->d_binRec) {
+        HIPCHK(hipSetDevice(a->device));
+        HIPCHK(hipDeviceSynchronize());
+    }
+    if (a->d_binOff) (void)hipFree(a->d_binOff);
+    if (a->d_binRec) (void)hipFree(a->d_binRec);
+    a->d_binOff = nullptr;
+    a->d_binRec = nullptr;
+    a->binRecords = 0;
+    return VBT_OK;
+}
+
+int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream)
+{
+    if (!a) return vbt_fail(VBT_ERR_INVALID, "vbt_asset_build_frame_index: null asset");
+    if (a->h.profileType == 2) return vbt_fail(VBT_ERR_INVALID, "frame index: render assets only");
+    int rc = vbt_asset_drop_frame_index(a);
+    if (rc) return rc;
+    const uint64_t C = a->controlsPerFrame, N = (uint64_t)INDEX_BINS * (C + 1u);
+    if (bytes) *bytes = 0;
+    if (a->A == 0 || C == 0) return VBT_OK;
+    if (N > 0x7FFFFFFFull) return vbt_fail(VBT_ERR_INVALID, "frame index: too many controls");
+    HIPCHK(hipSetDevice(a->device));
+    hipStream_t s = S(stream);
+    const AssetDev ad = a->dev();
+    uint32_t* cnt = nullptr;
+    void* tmp = nullptr;
+    size_t tmpBytes = 0;
+    uint32_t last = 0;
+    auto cleanup = [&]() {
+        if (cnt) (void)hipFree(cnt);
+        if (tmp) (void)hipFree(tmp);
+    };
+    if (hipMalloc((void**)&cnt, 4ull * N) != hipSuccess || hipMalloc((void**)&a->d_binOff, 4ull * N) != hipSuccess) {
+        cleanup();
+        vbt_asset_drop_frame_index(a);
+        return vbt_fail(VBT_ERR_NOMEM, "frame index: out of device memory");
+    }
+    k_index_count<<<blocks_for(C + 1u, 256), 256, 0, s>>>(ad, cnt, C);
+    // the record total must fit the 32-bit offsets (each prefix is at most it)
+    unsigned long long sum = 0;
+    if (hipMemsetAsync(a->d_err + 2, 0, 8, s) != hipSuccess) {
+        cleanup();
+        vbt_asset_drop_frame_index(a);
+        return vbt_fail(VBT_ERR_DEVICE, "frame index: clear failed");
+    }
+    k_sum64<<<1024, 256, 0, s>>>(cnt, N, (unsigned long long*)(a->d_err + 2));
+    if (hipMemcpyAsync(&sum, a->d_err + 2, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        cleanup();
+        vbt_asset_drop_frame_index(a);
+        return vbt_fail(VBT_ERR_DEVICE, "frame index: count failed");
+    }
+    if (sum > 0xFFFFFFFFull) {
+        cleanup();
+        vbt_asset_drop_frame_index(a);
+        return vbt_fail(VBT_ERR_INVALID, "frame index: more than 2^32 records");
+    }
+    if (sum == 0) {
+        cleanup();
+        return vbt_asset_drop_frame_index(a); // nothing indexable: builds keep the bracket search
+    }
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, cnt, a->d_binOff, (int)N, s) != hipSuccess ||
+        hipMalloc(&tmp, tmpBytes ? tmpBytes : 4) != hipSuccess ||
+        hipcub::DeviceScan::ExclusiveSum(tmp, tmpBytes, cnt, a->d_binOff, (int)N, s) != hipSuccess ||
+        hipMemcpyAsync(&last, a->d_binOff + (N - 1u), 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        cleanup();
+        vbt_asset_drop_frame_index(a);
+        return vbt_fail(VBT_ERR_DEVICE, "frame index: scan failed");
+    }
+    // the slot count after each bin's last slot is 0, so `last` is the total
+    const uint64_t total = last;
+    if (hipMalloc((void**)&a->d_binRec, total * sizeof(uint2)) != hipSuccess) {
+        cleanup();
+        vbt_asset_drop_frame_index(a);
+        return vbt_fail(VBT_ERR_NOMEM, "frame index: out of device memory");
+    }
+    k_index_scatter<<<blocks_for(C, 256), 256, 0, s>>>(ad, a->d_binOff, a->d_binRec, C);
+    const hipError_t e = hipGetLastError();
+    (void)hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) {
+        vbt_asset_drop_frame_index(a);
+        return vbt_fail(VBT_ERR_DEVICE, hipGetErrorString(e));
+    }
+    a->binRecords = total;
+    if (bytes) *bytes = 4ull * N + 8ull * total;
     return VBT_OK;
 }
 

@@ -159,6 +159,8 @@ def lib():
         L.vbt_march_rays.argtypes = [c_void_p, c_void_p, ctypes.POINTER(MarchParams), c_void_p, c_u32, c_void_p,
                                      c_void_p]
         L.vbt_cycles_blob_bytes.argtypes = [c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+        L.vbt_asset_build_frame_index.argtypes = [c_void_p, ctypes.POINTER(ctypes.c_uint64), c_void_p]
+        L.vbt_asset_drop_frame_index.argtypes = [c_void_p]
         L.vbt_cycles_pack_blob.argtypes = [c_void_p, c_int, c_void_p, ctypes.c_uint64, c_void_p]
         L.vbt_cycles_sample_density.argtypes = [c_void_p, ctypes.c_uint64, c_int, c_void_p, c_u32, c_void_p,
                                                 c_void_p]
@@ -480,6 +482,16 @@ class Asset:
                    stream: Stream | None = None):
         check(lib().vbt_march_rays(self.h, cache576.ptr, ctypes.byref(params), rays.ptr, n, out.ptr,
                                    (stream or self.stream).h))
+
+    def build_frame_index(self) -> int:
+        """Builds the frame index that one- and two-frame packed-cache builds
+        then read; returns its size in bytes."""
+        n = ctypes.c_uint64()
+        check(lib().vbt_asset_build_frame_index(self.h, ctypes.byref(n), self.stream.h))
+        return n.value
+
+    def drop_frame_index(self):
+        check(lib().vbt_asset_drop_frame_index(self.h))
 
     def cycles_blob(self, frame_index: int) -> DeviceArray:
         """The Cycles resident blob (header, offsets, payload) for one frame,
