@@ -481,7 +481,8 @@ int vbt_march_rays(vbt_asset a, const float* d_cache576, const vbt_march_params*
  * one or two frames in the packed layout read it, other builds and leaves
  * whose keys do not strictly increase keep the bracket search.  Costs
  * 4 * 8 * (controlsPerFrame + 1) bytes of offsets plus 8 bytes per indexed
- * record (*bytes); synchronizes the stream.  Render assets only. */
+ * record (*bytes); synchronizes the stream.  Render assets only.  Build and
+ * drop change the asset: call them while no other call on it runs. */
 int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream);
 /* Frees the frame index (waits for the device first). */
 int vbt_asset_drop_frame_index(vbt_asset a);
