@@ -452,19 +452,27 @@ __device__ __forceinline__ bool index_run(const AssetDev& a, uint64_t g, uint32_
     return true;
 }
 
-__global__ void __launch_bounds__(256) k_index_count(AssetDev a, uint32_t* cnt, uint64_t C)
+// counts per bin and slot; *wide is set when a run holds fp32 records (the
+// index then stores 8-byte records, else 4-byte {frame, half bits})
+__global__ void __launch_bounds__(256) k_index_count(AssetDev a, uint32_t* cnt, uint64_t C, uint32_t* wide)
 {
     const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (g > C) return;
+    bool fp32 = false;
     for (uint32_t b = 0; b < INDEX_BINS; ++b) {
-        const uint32_t*W;
+        const uint32_t* W;
         StreamRef s;
         uint32_t k0, lo, hi;
-        cnt[b * (C + 1u) + g] = g < C && index_run(a, g, b, W, s, k0, lo, hi) ? hi - lo + 1u : 0u;
+        const bool run = g < C && index_run(a, g, b, W, s, k0, lo, hi);
+        cnt[b * (C + 1u) + g] = run ? hi - lo + 1u : 0u;
+        fp32 = fp32 || (run && s.codec == CODEC_FP32);
     }
+    if (fp32) atomicOr(wide, 1u);
 }
 
-__global__ void __launch_bounds__(256) k_index_scatter(AssetDev a, const uint32_t* off, uint2* rec, uint64_t C)
+// WIDE: {frame, value bits}; else {frame | half bits << 16} (fp16 records only)
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_index_scatter(AssetDev a, const uint32_t* off, void* rec, uint64_t C)
 {
     const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (g >= C) return;
@@ -473,14 +481,34 @@ __global__ void __launch_bounds__(256) k_index_scatter(AssetDev a, const uint32_
         StreamRef s;
         uint32_t k0, lo, hi;
         if (!index_run(a, g, b, W, s, k0, lo, hi)) continue;
-        uint2* o = rec + off[b * (C + 1u) + g];
+        const uint32_t at = off[b * (C + 1u) + g];
         const uint32_t kb = keyframe_bytes(s.codec);
         for (uint32_t k = lo; k <= hi; ++k) {
-            int f;
-            float v;
-            w_rec(W, k0 + k * kb, s.codec, f, v);
-            o[k - lo] = make_uint2((uint32_t)f, __float_as_uint(v));
+            const uint32_t ro = k0 + k * kb;
+            if constexpr (WIDE) {
+                int f;
+                float v;
+                w_rec(W, ro, s.codec, f, v);
+                ((uint2*)rec)[at + k - lo] = make_uint2((uint32_t)f, __float_as_uint(v));
+            } else {
+                const uint32_t x = __builtin_amdgcn_alignbyte(W[(ro >> 2) + 1u], W[ro >> 2], ro & 3u);
+                ((uint32_t*)rec)[at + k - lo] = (x & 0xFFu) | (((x >> 8) & 0xFFFFu) << 16);
+            }
         }
+    }
+}
+
+template <bool WIDE>
+__device__ __forceinline__ void index_rec(const void* rec, uint32_t j, int& f, float& v)
+{
+    if constexpr (WIDE) {
+        const uint2 r = ((const uint2*)rec)[j];
+        f = (int)r.x;
+        v = __uint_as_float(r.y);
+    } else {
+        const uint32_t r = ((const uint32_t*)rec)[j];
+        f = (int)(r & 0xFFFFu);
+        v = half_to_float(r >> 16);
     }
 }
 
@@ -496,7 +524,8 @@ __global__ void __launch_bounds__(256) k_sum64(const uint32_t* cnt, uint64_t n, 
 // Packed cache, nt <= 2 frames, from the frame index: the same values as
 // k_cache_exact (for strictly increasing keys the reference bracket is the
 // last key at or before the frame and the next one).
-__global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_t* off, const uint2* rec, int t0,
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_t* off, const void* rec, int t0,
                                                        int nt, float* out, uint64_t total, uint32_t* errFlag)
 {
     const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
@@ -512,21 +541,23 @@ __global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_
             if (st) atomicOr(errFlag, 1u);
         } else {
             uint32_t j = o0;
-            uint2 r = rec[j];
-            if (t < (int)r.x) {
-                val = __uint_as_float(r.y);
+            int f0, f1 = 0;
+            float v0, v1 = 0.0f;
+            index_rec<WIDE>(rec, j, f0, v0);
+            if (t < f0) {
+                val = v0;
             } else {
-                uint2 n = j + 1u < o1 ? rec[j + 1u] : r;
-                while (j + 1u < o1 && (int)n.x <= t) {
+                if (j + 1u < o1) index_rec<WIDE>(rec, j + 1u, f1, v1);
+                while (j + 1u < o1 && f1 <= t) {
                     ++j;
-                    r = n;
-                    if (j + 1u < o1) n = rec[j + 1u];
+                    f0 = f1;
+                    v0 = v1;
+                    if (j + 1u < o1) index_rec<WIDE>(rec, j + 1u, f1, v1);
                 }
-                if ((int)r.x == t || j + 1u == o1) {
-                    val = __uint_as_float(r.y);
+                if (f0 == t || j + 1u == o1) {
+                    val = v0;
                 } else {
-                    const float v0 = __uint_as_float(r.y), v1 = __uint_as_float(n.y);
-                    const float alpha = (float)(t - (int)r.x) / (float)((int)n.x - (int)r.x);
+                    const float alpha = (float)(t - f0) / (float)(f1 - f0);
                     val = v0 + alpha * (v1 - v0);
                 }
             }
@@ -850,8 +881,9 @@ struct vbt_asset_s {
     int cacheTile = CACHE_TILE_DEFAULT;
     Pair* d_blocks = nullptr;
     uint32_t* d_binOff = nullptr; // frame index: record offsets, INDEX_BINS x (controlsPerFrame + 1)
-    uint2* d_binRec = nullptr;    // frame index: {frame, value bits} records
+    void* d_binRec = nullptr;     // frame index records (binWide: 8-byte {frame, value bits}, else 4-byte)
     uint64_t binRecords = 0;
+    bool binWide = false;
     uint32_t A = 0, bad = 0;
     uint64_t controlsPerFrame = 0, payloadBytes = 0, deviceBytes = 0, fast = 0;
     uint64_t modeCounts[4] = {0, 0, 0, 0};
@@ -1114,8 +1146,11 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     if (a->h.profileType == 2) {
         k_sci_leaf<false><<<dim3(a->A, (uint32_t)nt), 512, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
     } else if (layout == VBT_CACHE_PACKED) {
-        if (nt <= CACHE_EXACT_MAX_FRAMES && a->d_binOff)
-            k_cache_indexed<<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
+        if (nt <= CACHE_EXACT_MAX_FRAMES && a->d_binOff && a->binWide)
+            k_cache_indexed<true><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
+                ad, a->d_binOff, a->d_binRec, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        else if (nt <= CACHE_EXACT_MAX_FRAMES && a->d_binOff)
+            k_cache_indexed<false><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
                 ad, a->d_binOff, a->d_binRec, t0, nt, d_out, a->controlsPerFrame, a->d_err);
         else if (nt <= CACHE_EXACT_MAX_FRAMES)
             k_cache_exact<VBT_CACHE_PACKED><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
@@ -1238,6 +1273,7 @@ int vbt_asset_drop_frame_index(vbt_asset a)
     a->d_binOff = nullptr;
     a->d_binRec = nullptr;
     a->binRecords = 0;
+    a->binWide = false;
     return VBT_OK;
 }
 
@@ -1267,16 +1303,19 @@ int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream)
         vbt_asset_drop_frame_index(a);
         return vbt_fail(VBT_ERR_NOMEM, "frame index: out of device memory");
     }
-    k_index_count<<<blocks_for(C + 1u, 256), 256, 0, s>>>(ad, cnt, C);
-    // the record total must fit the 32-bit offsets (each prefix is at most it)
-    unsigned long long sum = 0;
-    if (hipMemsetAsync(a->d_err + 2, 0, 8, s) != hipSuccess) {
+    // d_err[1]: the fp32 flag, d_err[2..3]: the 64-bit record total
+    if (hipMemsetAsync(a->d_err + 1, 0, 12, s) != hipSuccess) {
         cleanup();
         vbt_asset_drop_frame_index(a);
         return vbt_fail(VBT_ERR_DEVICE, "frame index: clear failed");
     }
+    k_index_count<<<blocks_for(C + 1u, 256), 256, 0, s>>>(ad, cnt, C, a->d_err + 1);
+    // the record total must fit the 32-bit offsets (each prefix is at most it)
+    unsigned long long sum = 0;
+    uint32_t wide = 0;
     k_sum64<<<1024, 256, 0, s>>>(cnt, N, (unsigned long long*)(a->d_err + 2));
     if (hipMemcpyAsync(&sum, a->d_err + 2, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&wide, a->d_err + 1, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         cleanup();
         vbt_asset_drop_frame_index(a);
@@ -1302,12 +1341,14 @@ int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream)
     }
     // the slot count after each bin's last slot is 0, so `last` is the total
     const uint64_t total = last;
-    if (hipMalloc((void**)&a->d_binRec, total * sizeof(uint2)) != hipSuccess) {
+    const uint64_t recBytes = wide ? 8u : 4u;
+    if (hipMalloc(&a->d_binRec, total * recBytes) != hipSuccess) {
         cleanup();
         vbt_asset_drop_frame_index(a);
         return vbt_fail(VBT_ERR_NOMEM, "frame index: out of device memory");
     }
-    k_index_scatter<<<blocks_for(C, 256), 256, 0, s>>>(ad, a->d_binOff, a->d_binRec, C);
+    if (wide) k_index_scatter<true><<<blocks_for(C, 256), 256, 0, s>>>(ad, a->d_binOff, a->d_binRec, C);
+    else k_index_scatter<false><<<blocks_for(C, 256), 256, 0, s>>>(ad, a->d_binOff, a->d_binRec, C);
     const hipError_t e = hipGetLastError();
     (void)hipStreamSynchronize(s);
     cleanup();
@@ -1316,7 +1357,8 @@ int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream)
         return vbt_fail(VBT_ERR_DEVICE, hipGetErrorString(e));
     }
     a->binRecords = total;
-    if (bytes) *bytes = 4ull * N + 8ull * total;
+    a->binWide = wide != 0;
+    if (bytes) *bytes = 4ull * N + recBytes * total;
     return VBT_OK;
 }
 
