@@ -217,17 +217,44 @@ class OracleFile:
         return np.array([oracle().vbto_leaf_cache_count(ctypes.byref(self.vf), l)
                          for l in range(self.header.leafCount)], dtype=np.uint32)
 
+    def packed_offsets(self) -> np.ndarray:
+        """Packed-cache slot offset of every active leaf (+ the total)."""
+        if getattr(self, "_offs", None) is None:
+            compact, n = self.page_table()
+            counts = self.cache_counts()
+            active = counts[compact != 0xFFFFFFFF] if self.header.profileType != 2 else counts
+            offs = np.zeros(len(active) + 1, np.uint64)
+            offs[1:] = np.cumsum(active, dtype=np.uint64)
+            self._offs = offs
+        return self._offs
+
     def cache_packed(self, t: int):
-        compact, n = self.page_table()
-        counts = self.cache_counts()
-        active = counts[compact != 0xFFFFFFFF] if self.header.profileType != 2 else counts
-        offs = np.zeros(len(active) + 1, np.uint64)
-        offs[1:] = np.cumsum(active, dtype=np.uint64)
+        offs = self.packed_offsets()
         out = np.zeros(int(offs[-1]), np.float32)
         rc = oracle().vbto_frame_cache_packed(ctypes.byref(self.vf), t, offs.ctypes.data, out.ctypes.data)
         if rc != 0:
             raise ValueError(last_error())
         return out, offs
+
+    def cache_packed_frames(self, ts, threads: int = 8) -> np.ndarray:
+        """Packed caches of frames `ts` as one (len(ts), slots) array, one
+        frame per host thread (the oracle call releases the GIL)."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        offs = self.packed_offsets()
+        ts = list(ts)
+        out = np.zeros((len(ts), int(offs[-1])), np.float32)
+
+        def one(i):
+            row = out[i]
+            rc = oracle().vbto_frame_cache_packed(ctypes.byref(self.vf), int(ts[i]), offs.ctypes.data,
+                                                  row.ctypes.data)
+            if rc != 0:
+                raise ValueError(last_error())
+
+        with ThreadPoolExecutor(max(1, threads)) as pool:
+            list(pool.map(one, range(len(ts))))
+        return out
 
     def cache_576(self, t: int):
         compact, n = self.page_table()

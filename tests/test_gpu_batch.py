@@ -129,6 +129,31 @@ def test_every_tile_shape(world, tile):
     b.close()
 
 
+@pytest.mark.parametrize("tile", range(10))
+def test_one_shot_kernel_every_tile_shape(world, tile):
+    """The one-shot frame-cache kernel (vbt_decode_frame_cache, the kernel the
+    benchmark times; its staging depth differs from the batch kernels') at
+    every tile shape the autotuner may pick, on every render job of the
+    world: fp32 bursts, permuted keys, Mode2 shells, both layouts."""
+    assets, jobs, expected = world
+    for (name, t0, nt, layout), exp in zip(jobs, expected):
+        a = assets[name][0]
+        if a.header.profileType == 2:
+            continue  # scientific caches have one kernel shape
+        assert nt > 2  # more than two frames: the staged sweep kernel, not the exact one
+        n = a.info.controlsPerFrame if layout == hip.CACHE_PACKED else a.info.activeLeaves * 576
+        out = hip.DeviceArray((nt, n), np.float32, a.stream)
+        out.fill_bytes(0xFF)
+        keep = a.cache_tile
+        a.cache_tile = tile
+        try:
+            a.frame_cache_into(out, t0, nt, layout)
+        finally:
+            a.cache_tile = keep
+        bitexact(out.host(), exp, f"one-shot tile {tile}: {name} [{t0},{t0 + nt}) layout {layout}")
+        out.free()
+
+
 @pytest.mark.parametrize("kw", [
     dict(ranges=1, chunk=1, order=0, queues=1, persistent=1),
     dict(ranges=3, chunk=4, order=1, queues=8),

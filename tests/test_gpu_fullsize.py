@@ -1,21 +1,24 @@
-"""The benchmark workloads at full size, against the oracle: the Mode2 liquid
-level set (320^3 x 150 frames, every leaf active) with its phi sign
-checked on every |phi| < 2 voxel, and the fire triple (384x384x256 x 96
-frames each) built as one batch.  Sampled frames are compared bit for bit;
-the rest is covered by the smaller assets of the other GPU tests."""
-import os
-
+"""The benchmark workloads at full size, against the oracle:
+- smoke256 (256^3 x 120 frames): the 120-frame build exactly as the
+  benchmark times it (autotuned tile, one launch), every frame, and the same
+  build at every tile shape;
+- the Mode2 liquid level set (320^3 x 150 frames, every leaf active) with its
+  phi sign checked on every |phi| < 2 voxel;
+- the fire triple (384x384x256 x 96 frames each) built as one batch;
+- the scientific field sci192 (192^3 x 200 frames): dense frames, packed
+  caches and point queries.
+Frames not compared here are covered by the smaller assets of the other GPU
+tests."""
 import numpy as np
 import pytest
 
 import oracle_py
 from vbt_amd import synth
+from workloads import THREADS, encoded
 
 hip = pytest.importorskip("vbt_amd.hip")
 
 pytestmark = pytest.mark.gpu
-
-THREADS = min(16, os.cpu_count() or 1)
 
 
 def bitexact(a, b, what):
@@ -28,7 +31,7 @@ def bitexact(a, b, what):
 
 def test_liquid320_levelset_sign_and_parity():
     w = synth.WORKLOADS["liquid320"]
-    data = w.encode(threads=THREADS)
+    data = encoded("liquid320")
     a = hip.Asset(data)
     o = oracle_py.OracleFile(data)
     W, H, D, F = a.dims
@@ -60,7 +63,7 @@ def test_liquid320_levelset_sign_and_parity():
 
 def test_fire_triple_batch_parity():
     names = ["fire_density", "fire_flames", "fire_temperature"]
-    datas = {n: synth.WORKLOADS[n].encode(threads=THREADS) for n in names}
+    datas = {n: encoded(n) for n in names}
     assets = {n: hip.Asset(d) for n, d in datas.items()}
     jobs, outs = [], {}
     for n in names:
@@ -81,3 +84,63 @@ def test_fire_triple_batch_parity():
     b.close()
     for a in assets.values():
         a.close()
+
+
+def test_smoke256_benchmark_build_every_frame():
+    """The headline workload as the benchmark runs it: tile autotuned for the
+    120-frame launch, one launch over all frames; every frame of the output
+    equals the oracle's packed cache bit for bit.  Then the same launch at
+    every tile shape the autotuner can pick."""
+    data = encoded("smoke256")
+    a = hip.Asset(data)
+    o = oracle_py.OracleFile(data)
+    W, H, D, F = a.dims
+    assert (W, H, D, F) == (256, 256, 256, 120)
+    cpf = a.info.controlsPerFrame
+    exp = o.cache_packed_frames(range(F), threads=THREADS)
+    assert exp.shape == (F, cpf)
+    best, ms = a.autotune(F, reps=2)
+    out = hip.DeviceArray((F, cpf), np.float32, a.stream)
+    out.fill_bytes(0xFF)
+    a.frame_cache_into(out, 0, F)
+    bitexact(out.host(), exp, f"smoke256 120-frame build, autotuned tile {best}")
+    for tile in range(len(hip.Asset.tile_shapes())):
+        a.cache_tile = tile
+        out.fill_bytes(0xFF)
+        a.frame_cache_into(out, 0, F)
+        bitexact(out.host(), exp, f"smoke256 120-frame build, tile {tile}")
+    # a frame range in the middle of the sequence
+    a.frame_cache_into(out, 37, 50)
+    bitexact(out.host_slice(0, 50 * cpf).reshape(50, cpf), exp[37:87], "smoke256 frames 37..86")
+    out.free()
+    a.close()
+
+
+def test_sci192_dense_cache_and_queries():
+    """The scientific field of the serving mix at full size (192^3 x 200,
+    Grid4 DCT controls, sparse spikes): dense frames, packed caches of a
+    frame run and 200k point queries over all frames, bit for bit."""
+    data = encoded("sci192")
+    a = hip.Asset(data)
+    o = oracle_py.OracleFile(data)
+    W, H, D, F = a.dims
+    assert (W, H, D, F) == (192, 192, 192, 200) and a.header.profileType == 2
+    assert a.info.activeLeaves == a.header.leafCount == 13824
+    for t in (0, 101, 199):
+        bitexact(a.dense(t), o.frame(t, threads=THREADS), f"sci192 dense t={t}")
+    cpf = a.info.controlsPerFrame
+    out = hip.DeviceArray((6, cpf), np.float32, a.stream)
+    a.frame_cache_into(out, 97, 6)
+    exp = o.cache_packed_frames(range(97, 103), threads=THREADS)
+    bitexact(out.host(), exp, "sci192 packed cache frames 97..102")
+    one = a.frame_cache(199, 1)
+    bitexact(one[0], o.cache_packed(199)[0], "sci192 packed cache frame 199")
+    rng = np.random.default_rng(0x5C1192)
+    n = 200000
+    q = np.stack([rng.integers(0, W, n), rng.integers(0, H, n), rng.integers(0, D, n), rng.integers(0, F, n)],
+                 axis=1).astype(np.uint32)
+    exp_q = o.query(q, threads=THREADS)
+    bitexact(a.query(q), exp_q, "sci192 batched queries")
+    bitexact(a.query(q[:20000], plain=True), exp_q[:20000], "sci192 plain queries")
+    out.free()
+    a.close()
