@@ -312,6 +312,28 @@ int vbt_cache_batch_autotune(vbt_cache_batch b, int reps, vbt_batch_shape* best,
 int vbt_cache_batch_trial_count(vbt_cache_batch b);
 int vbt_cache_batch_trial(vbt_cache_batch b, int i, vbt_batch_shape* shape, float* ms);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the work of vbt_cache_batch_create + vbt_cache_batch_launch_direct
+ * in one call that allocates nothing.  The batch's work queue (job table,
+ * entries, queue bounds and counters) lives in a file-scope device array of
+ * VBT_NOALLOC_SCRATCH_BYTES that each device the library runs on holds until
+ * the process ends; it is written by kernels from kernel arguments (no
+ * memset, no host-to-device copy) and its counters are cleared on every run.
+ * Limits: at most VBT_NOALLOC_MAX_JOBS jobs and VBT_NOALLOC_MAX_TILES render
+ * leaf tiles (slots / tile slots, summed over the jobs); larger requests
+ * return VBT_ERR_INVALID.  The shape (NULL: vbt_batch_noalloc_default_shape)
+ * must have order 0 (job / leaf order, queues split evenly by tile count) and
+ * heavyTile -1; scientific jobs run on `stream` before the queue kernel.
+ * Outputs equal vbt_cache_batch_launch's bit for bit.  Every call on a device
+ * shares that device's array: calls that overlap on one device (other
+ * streams, other threads) race and must be serialised by the caller. */
+#define VBT_NOALLOC_MAX_JOBS 32
+#define VBT_NOALLOC_MAX_TILES (1u << 19)
+#define VBT_NOALLOC_SCRATCH_BYTES (4u * 1024u * 1024u + 8u * 1024u)
+void vbt_batch_noalloc_default_shape(vbt_batch_shape* shape);
+int vbt_cache_batch_run_noalloc(const vbt_cache_job* jobs, int nJobs, const vbt_batch_shape* shape, void* stream);
+
 /* --------------------------------------------------- voxels & queries -- */
 enum {
     VBT_SIGN_NONE = 0,        /* render_temporal_vbt_to_vdb default (optional shellEmptyScale) */

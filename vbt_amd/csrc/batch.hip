@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -480,6 +481,27 @@ int capture(vbt_cache_batch b)
     return VBT_OK;
 }
 
+// One job of a batch: an asset on device `dev`, a frame range inside it and
+// a layout it supports.  Sets the job's kind.
+int check_job(const vbt_cache_job& job, int dev, uint32_t& kind)
+{
+    if (!job.asset || !job.d_out) return vbt_fail(VBT_ERR_INVALID, "cache batch job without asset or output");
+    if (vbt_internal_asset_device(job.asset) != dev)
+        return vbt_fail(VBT_ERR_INVALID, "cache batch asset lives on another device");
+    const vbt_file_header* h = vbt_internal_asset_header(job.asset);
+    if (job.nt <= 0 || job.t0 < 0 || (uint64_t)job.t0 + (uint64_t)job.nt > h->frames)
+        return vbt_fail(VBT_ERR_INVALID, "frame range outside the asset");
+    if (job.layout != VBT_CACHE_PACKED && job.layout != VBT_CACHE_STRIDE576)
+        return vbt_fail(VBT_ERR_INVALID, "unknown cache layout");
+    if (h->profileType == 2 && job.layout != VBT_CACHE_PACKED)
+        return vbt_fail(VBT_ERR_INVALID, "scientific assets support the packed cache only");
+    const AssetDev ad = vbt_internal_asset_dev(job.asset);
+    if (4ull * (job.layout == VBT_CACHE_PACKED ? ad.controlsPerFrame : (uint64_t)ad.A * 576u) > 0xFFFFFFFFull)
+        return vbt_fail(VBT_ERR_INVALID, "frame cache: one frame exceeds 4 GiB");
+    kind = h->profileType == 2 ? JOB_SCI : (job.layout == VBT_CACHE_PACKED ? JOB_PACKED : JOB_576);
+    return VBT_OK;
+}
+
 int report(vbt_cache_batch b)
 {
     for (const vbt_cache_job& j : b->jobs)
@@ -517,22 +539,11 @@ int vbt_cache_batch_create(const vbt_cache_job* jobs, int nJobs, const vbt_batch
         return rc;
     };
     for (int j = 0; j < nJobs; ++j) {
-        const vbt_cache_job& job = jobs[j];
-        if (!job.asset || !job.d_out) return fail(vbt_fail(VBT_ERR_INVALID, "cache batch job without asset or output"));
-        if (vbt_internal_asset_device(job.asset) != dev)
-            return fail(vbt_fail(VBT_ERR_INVALID, "cache batch asset lives on another device"));
-        const vbt_file_header* h = vbt_internal_asset_header(job.asset);
-        if (job.nt <= 0 || job.t0 < 0 || (uint64_t)job.t0 + (uint64_t)job.nt > h->frames)
-            return fail(vbt_fail(VBT_ERR_INVALID, "frame range outside the asset"));
-        if (job.layout != VBT_CACHE_PACKED && job.layout != VBT_CACHE_STRIDE576)
-            return fail(vbt_fail(VBT_ERR_INVALID, "unknown cache layout"));
-        if (h->profileType == 2 && job.layout != VBT_CACHE_PACKED)
-            return fail(vbt_fail(VBT_ERR_INVALID, "scientific assets support the packed cache only"));
-        const AssetDev ad = vbt_internal_asset_dev(job.asset);
-        if (4ull * (job.layout == VBT_CACHE_PACKED ? ad.controlsPerFrame : (uint64_t)ad.A * 576u) > 0xFFFFFFFFull)
-            return fail(vbt_fail(VBT_ERR_INVALID, "frame cache: one frame exceeds 4 GiB"));
-        b->jobs.push_back(job);
-        b->kinds.push_back(h->profileType == 2 ? JOB_SCI : (job.layout == VBT_CACHE_PACKED ? JOB_PACKED : JOB_576));
+        uint32_t kind = 0;
+        const int rc = check_job(jobs[j], dev, kind);
+        if (rc) return fail(rc);
+        b->jobs.push_back(jobs[j]);
+        b->kinds.push_back(kind);
     }
     hipDeviceProp_t prop;
     {
@@ -768,6 +779,213 @@ int vbt_cache_batch_trial(vbt_cache_batch b, int i, vbt_batch_shape* shape, floa
     if (!b || i < 0 || i >= (int)b->trials.size()) return vbt_fail(VBT_ERR_INVALID, "trial index out of range");
     if (shape) *shape = b->trials[(size_t)i].first;
     if (ms) *ms = b->trials[(size_t)i].second;
+    return VBT_OK;
+}
+
+} // extern "C"
+
+// ----------------------------------------------- batch run, no allocation --
+// vbt_cache_batch_run_noalloc: the batch's work queue (job table, entries,
+// queue bounds, queue counters) lives in the file-scope device array
+// `scratch` instead of memory allocated per batch, and is written by two
+// small kernels from kernel arguments, so a run calls no hipMalloc / hipFree
+// and no host-to-device copy.  Entries are in job / leaf order (order 0) and
+// the queues split them evenly by tile count; the decode kernels are the
+// batch's own.
+namespace {
+
+constexpr uint32_t NA_MAX_JOBS = VBT_NOALLOC_MAX_JOBS;
+constexpr uint32_t NA_MAX_TILES = VBT_NOALLOC_MAX_TILES;
+constexpr size_t na_round(size_t n) { return (n + 255u) & ~(size_t)255u; }
+constexpr size_t NA_JOBS_OFF = 0;
+constexpr size_t NA_BOUNDS_OFF = NA_JOBS_OFF + na_round(NA_MAX_JOBS * sizeof(BatchJob));
+constexpr size_t NA_COUNTERS_OFF = NA_BOUNDS_OFF + na_round(2 * MAX_QUEUES * sizeof(uint32_t));
+constexpr size_t NA_ENTRIES_OFF = NA_COUNTERS_OFF + na_round(MAX_QUEUES * COUNTER_STRIDE * sizeof(uint32_t));
+constexpr size_t NA_SCRATCH_BYTES = NA_ENTRIES_OFF + (size_t)NA_MAX_TILES * sizeof(uint2);
+static_assert(NA_SCRATCH_BYTES <= VBT_NOALLOC_SCRATCH_BYTES, "scratch layout exceeds the stated maximum");
+
+// One copy per device, held until the process ends.  Host code never uses
+// this name as an address: it takes the device address from
+// hipGetSymbolAddress (na_scratch below).
+static __device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+constexpr uint32_t NA_CHUNK = 8; // job records per setup launch (kernel arguments)
+
+struct NaJobChunk {
+    BatchJob jobs[NA_CHUNK];
+    uint32_t first, count;
+};
+
+struct NaPlan {
+    uint32_t prefix[NA_MAX_JOBS + 1]; // first tile of each job in the entry order
+    uint32_t bounds[2 * MAX_QUEUES];
+    uint32_t nJobs, tiles;
+};
+
+__global__ void k_na_jobs(NaJobChunk c, BatchJob* dst)
+{
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (uint32_t j = 0; j < NA_CHUNK; ++j)
+        if (j < c.count) dst[c.first + j] = c.jobs[j];
+}
+
+// Entries (job, tile) in job / leaf order, the queue bounds, and the queue
+// counters cleared (every run: the array keeps the previous run's values).
+__global__ void __launch_bounds__(256) k_na_entries(NaPlan p, uint2* entries, uint32_t* bounds, uint32_t* counters)
+{
+    if (blockIdx.x == 0) {
+        for (uint32_t k = threadIdx.x; k < MAX_QUEUES * COUNTER_STRIDE; k += 256u) counters[k] = 0u;
+        if (threadIdx.x < 2u * MAX_QUEUES) bounds[threadIdx.x] = p.bounds[threadIdx.x];
+    }
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= p.tiles) return;
+    uint32_t j = 0;
+    while (j + 1u < p.nJobs && p.prefix[j + 1u] <= i) ++j;
+    entries[i] = make_uint2(j, i - p.prefix[j]);
+}
+
+struct NaDevice {
+    uint8_t* ptr = nullptr; // device address of `scratch` on this device
+    uint32_t numCU = 0;
+};
+
+std::mutex naMu;
+NaDevice naDev[64];
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current (each device holds its own copy).
+int na_scratch(int dev, NaDevice& out)
+{
+    if (dev < 0 || dev >= 64) return vbt_fail(VBT_ERR_INVALID, "no-alloc batch: device index above 63");
+    std::lock_guard<std::mutex> g(naMu);
+    NaDevice& d = naDev[dev];
+    if (!d.ptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return vbt_fail(VBT_ERR_DEVICE, std::string("no-alloc batch: hipGetSymbolAddress failed: ") +
+                                                hipGetErrorString(e));
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+            return vbt_fail(VBT_ERR_DEVICE, "no-alloc batch: compute unit count");
+        d.numCU = (uint32_t)cu;
+        d.ptr = (uint8_t*)p;
+    }
+    out = d;
+    return VBT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void vbt_batch_noalloc_default_shape(vbt_batch_shape* s)
+{
+    if (!s) return;
+    vbt_batch_default_shape(s);
+    s->order = 0;
+}
+
+int vbt_cache_batch_run_noalloc(const vbt_cache_job* jobs, int nJobs, const vbt_batch_shape* shape, void* stream)
+{
+    if (!jobs || nJobs <= 0) return vbt_fail(VBT_ERR_INVALID, "invalid cache batch");
+    if (nJobs > (int)NA_MAX_JOBS)
+        return vbt_fail(VBT_ERR_INVALID, "no-alloc batch: more than " + std::to_string(NA_MAX_JOBS) + " jobs");
+    vbt_batch_shape sh;
+    vbt_batch_noalloc_default_shape(&sh);
+    if (shape) sh = *shape;
+    int rc = check_shape(sh);
+    if (rc) return rc;
+    if (sh.order != 0 || sh.heavyTile != -1)
+        return vbt_fail(VBT_ERR_INVALID, "no-alloc batch: shape needs order 0 (job / leaf order) and heavyTile -1");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    NaDevice nd;
+    if ((rc = na_scratch(dev, nd))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t BLOCK = CACHE_TILES[sh.tile].block, R = (uint32_t)sh.ranges;
+    BatchJob hj[NA_MAX_JOBS];
+    uint32_t kinds[NA_MAX_JOBS];
+    NaPlan plan;
+    std::memset(&plan, 0, sizeof(plan));
+    uint64_t tiles = 0;
+    for (int j = 0; j < nJobs; ++j) {
+        const vbt_cache_job& job = jobs[j];
+        if ((rc = check_job(job, dev, kinds[j]))) return rc;
+        const AssetDev a = vbt_internal_asset_dev(job.asset);
+        BatchJob& J = hj[j];
+        J.a = a;
+        J.out = job.d_out;
+        J.err = vbt_internal_asset_err(job.asset);
+        J.t0 = job.t0;
+        J.nt = job.nt;
+        J.per = (job.nt + (int)R - 1) / (int)R;
+        J.kind = kinds[j];
+        plan.prefix[j] = (uint32_t)std::min<uint64_t>(tiles, NA_MAX_TILES);
+        if (a.A != 0 && J.kind != JOB_SCI) tiles += (a.controlsPerFrame + BLOCK - 1u) / BLOCK;
+    }
+    if (tiles > NA_MAX_TILES)
+        return vbt_fail(VBT_ERR_INVALID, "no-alloc batch: " + std::to_string(tiles) + " leaf tiles of " +
+                                             std::to_string(BLOCK) + " slots exceed the scratch maximum of " +
+                                             std::to_string(NA_MAX_TILES));
+    plan.prefix[nJobs] = (uint32_t)tiles;
+    plan.nJobs = (uint32_t)nJobs;
+    plan.tiles = (uint32_t)tiles;
+    const uint32_t Q = (uint32_t)sh.queues;
+    for (uint32_t q = 0; q < Q; ++q) {
+        plan.bounds[2 * q] = (uint32_t)(tiles * q / Q) * R;
+        plan.bounds[2 * q + 1] = (uint32_t)(tiles * (q + 1u) / Q) * R;
+    }
+    const uint64_t items = tiles * R;
+    BatchJob* dJobs = (BatchJob*)(nd.ptr + NA_JOBS_OFF);
+    uint32_t* dBounds = (uint32_t*)(nd.ptr + NA_BOUNDS_OFF);
+    uint32_t* dCounters = (uint32_t*)(nd.ptr + NA_COUNTERS_OFF);
+    uint2* dEntries = (uint2*)(nd.ptr + NA_ENTRIES_OFF);
+    for (uint32_t f = 0; f < (uint32_t)nJobs; f += NA_CHUNK) {
+        NaJobChunk c;
+        std::memset(&c, 0, sizeof(c));
+        c.first = f;
+        c.count = std::min<uint32_t>(NA_CHUNK, (uint32_t)nJobs - f);
+        for (uint32_t i = 0; i < c.count; ++i) c.jobs[i] = hj[f + i];
+        k_na_jobs<<<1, 64, 0, s>>>(c, dJobs);
+    }
+    k_na_entries<<<(uint32_t)std::max<uint64_t>(1u, (tiles + 255u) / 256u), 256, 0, s>>>(plan, dEntries, dBounds,
+                                                                                        dCounters);
+    HIPCHK(hipGetLastError());
+    for (int j = 0; j < nJobs; ++j) {
+        if (kinds[j] != JOB_576) continue;
+        const uint64_t n = (uint64_t)hj[j].a.A * 576u * (uint64_t)jobs[j].nt;
+        if (n) k_fill_f32<<<(uint32_t)std::min<uint64_t>((n + 255u) / 256u, 65535u), 256, 0, s>>>(jobs[j].d_out, n);
+    }
+    for (int j = 0; j < nJobs; ++j) {
+        if (kinds[j] != JOB_SCI) continue;
+        rc = vbt_decode_frame_cache(jobs[j].asset, jobs[j].t0, jobs[j].nt, VBT_CACHE_PACKED, jobs[j].d_out, stream);
+        if (rc != VBT_OK && rc != VBT_ERR_FORMAT) return rc; // poisoned leaves are reported after the launch
+    }
+    if (items) {
+        int occ = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel(sh.tile, sh.persistent != 0),
+                                                            (int)BLOCK, 0));
+        if (occ < 1) occ = 1;
+        const uint32_t perCU = sh.wgPerCU ? std::min<uint32_t>((uint32_t)sh.wgPerCU, (uint32_t)occ) : (uint32_t)occ;
+        const uint64_t claims = (items + (uint64_t)sh.chunk - 1u) / (uint64_t)sh.chunk;
+        const uint32_t grid =
+            sh.persistent ? (uint32_t)std::min<uint64_t>((uint64_t)nd.numCU * perCU, claims) : (uint32_t)claims;
+        BatchArgs A;
+        A.jobs = dJobs;
+        A.entries = dEntries;
+        A.bounds = dBounds;
+        A.counters = dCounters;
+        A.queues = Q;
+        A.ranges = R;
+        A.chunk = (uint32_t)sh.chunk;
+        A.items = (uint32_t)items;
+        launch_batch(sh.tile, sh.persistent != 0, grid, A, s);
+        HIPCHK(hipGetLastError());
+    }
+    for (int j = 0; j < nJobs; ++j)
+        if (vbt_internal_asset_bad(jobs[j].asset)) return vbt_internal_report_bad(jobs[j].asset, "frame cache");
     return VBT_OK;
 }
 

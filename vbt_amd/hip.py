@@ -206,6 +206,10 @@ def lib():
         L.vbt_cache_batch_trial_count.argtypes = [c_void_p]
         L.vbt_cache_batch_trial.argtypes = [c_void_p, c_int, ctypes.POINTER(BatchShape),
                                             ctypes.POINTER(ctypes.c_float)]
+        L.vbt_batch_noalloc_default_shape.argtypes = [ctypes.POINTER(BatchShape)]
+        L.vbt_batch_noalloc_default_shape.restype = None
+        L.vbt_cache_batch_run_noalloc.argtypes = [ctypes.POINTER(CacheJob), c_int, ctypes.POINTER(BatchShape),
+                                                  c_void_p]
         L.vbt_half_bits_to_float.restype = ctypes.c_float
         L.vbt_half_bits_to_float.argtypes = [ctypes.c_uint16]
         L.vbt_float_to_half_bits.restype = ctypes.c_uint16
@@ -564,6 +568,24 @@ def default_batch_shape() -> BatchShape:
     s = BatchShape()
     lib().vbt_batch_default_shape(ctypes.byref(s))
     return s
+
+
+def noalloc_batch_shape() -> BatchShape:
+    s = BatchShape()
+    lib().vbt_batch_noalloc_default_shape(ctypes.byref(s))
+    return s
+
+
+def cache_batch_run_noalloc(jobs, shape: BatchShape | None = None, stream: Stream | None = None):
+    """The batch's frame-cache builds in one call that allocates no device
+    memory (vbt_cache_batch_run_noalloc).  jobs: (asset, t0, nt, layout,
+    DeviceArray out)."""
+    jobs = list(jobs)
+    arr = (CacheJob * max(1, len(jobs)))()
+    for i, (asset, t0, nt, layout, out) in enumerate(jobs):
+        arr[i] = CacheJob(asset.h, int(t0), int(nt), int(layout), out.ptr)
+    check(lib().vbt_cache_batch_run_noalloc(arr, len(jobs), ctypes.byref(shape) if shape else None,
+                                            stream.h if stream else None))
 
 
 class CacheBatch:
