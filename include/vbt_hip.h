@@ -458,8 +458,7 @@ int vbt_device_fill(void* d_dst, uint32_t word, size_t bytes, void* stream);
  * (frame - f0) / (f1 - f0), 0 <= frame - f0 < f1 - f0 <= maxGap, for which the
  * frame-cache sweep's reciprocal-based weight differs from the correctly
  * rounded float quotient the reference computes (all 2^31 of them for
- * maxGap = 65535), plus the gaps whose reciprocal is not correctly rounded;
- * expected 0.  Synchronises `stream`. */
+ * maxGap = 65535); expected 0.  Synchronises `stream`. */
 int vbt_selftest_weights(uint32_t maxGap, uint64_t* mismatches, void* stream);
 /* ---- sampling and ray marching over a frame cache --------------------------
  * The reference viewer's sampler (VBTStudio/frontend/shaders/vbt_sample.comp:

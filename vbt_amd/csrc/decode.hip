@@ -832,8 +832,8 @@ __global__ void __launch_bounds__(256) k_fill16(uint4* dst, uint4 v, uint64_t n)
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) dst[i] = v;
 }
 
-// Every lerp weight the sweep can form: block g-1 checks gap g, its
-// reciprocal against RN32(1 / g) and each weight a / g, 0 <= a < g, against
+// Every lerp weight the sweep can form: block g-1 checks each weight a / g,
+// 0 <= a < g, of gap g (formed from the sweep's reciprocal of g) against
 // RN32(a / g).  The expected values are double quotients rounded to float:
 // a quotient a / g with g < 2^16 that is not itself a float lies at least
 // 2^-41 (relative) from every float rounding boundary (a boundary is a 25-bit
@@ -846,9 +846,9 @@ __global__ void __launch_bounds__(256) k_selftest_weights(uint32_t maxGap, unsig
     if (g > maxGap) return;
     const float b = (float)g;
     const float y = sweep_recip(b);
-    uint32_t bad = threadIdx.x == 0u && __float_as_uint(y) != __float_as_uint((float)(1.0 / (double)g)) ? 1u : 0u;
+    uint32_t bad = 0u;
     for (uint32_t a = threadIdx.x; a < g; a += 256u) {
-        const float got = sweep_alpha((int)a, b, y);
+        const float got = sweep_alpha((float)a, b, y);
         const float exp = (float)((double)a / (double)g);
         bad += __float_as_uint(got) != __float_as_uint(exp) ? 1u : 0u;
     }

@@ -438,11 +438,11 @@ __device__ inline float cache_slot_exact(const AssetDev& a, uint64_t g, int t, i
 // 1990) yields it in float arithmetic: q = RN32(a y) is within one ulp of
 // a / b, the residual r = a - b q is exact in one FMA, and RN32(q + r y) =
 // RN32(a / b).  b and a are exact floats (< 2^24).  Checked for every pair
-// with y from sweep_recip on the device (vbt_selftest_weights) and with y =
-// RN32(1 / b) on the host (tests/native/alpha_identity.c).
-__device__ __forceinline__ float sweep_alpha(int num, float b, float y)
+// with y = RN32(1 / b) on the host (tests/native/alpha_identity.c), and on
+// the device with y from sweep_recip, the bare hardware reciprocal, for every
+// pair 0 <= a < b < 2^16 (vbt_selftest_weights).
+__device__ __forceinline__ float sweep_alpha(float a, float b, float y)
 {
-    const float a = (float)num;
     const float q = a * y;
     const float r = __builtin_fmaf(-b, q, a);
     return __builtin_fmaf(r, y, q);
@@ -477,19 +477,18 @@ struct SweepState {
     uint32_t k;
     int fk, fn;
     float vk, vn;
-    float gap, y; // fn - fk and RN32(1 / (fn - fk))
+    float gap, y; // fn - fk and sweep_recip(fn - fk)
 };
 
-// RN32(1 / b) for an integer key gap 1 <= b < 2^16: the hardware float
-// reciprocal (within one ulp) refined by one Newton step with FMAs.  The
-// result is correctly rounded for every such b (vbt_selftest_weights checks
-// each one); a gap of 0 (never used for a weight) gives a non-finite value.
-__device__ __forceinline__ float sweep_recip(float b)
-{
-    const float y = __builtin_amdgcn_rcpf(b);
-    const float e = __builtin_fmaf(-b, y, 1.0f);
-    return __builtin_fmaf(e, y, y);
-}
+// Reciprocal of an integer key gap 1 <= b < 2^16 for sweep_alpha: the bare
+// hardware float reciprocal (v_rcp_f32, within one ulp).  It differs from
+// RN32(1 / b) for 6,450 of these gaps, yet with it Markstein's correction
+// still returns RN32(a / b) for every numerator 0 <= a < b of every gap: the
+// device self-test (vbt_selftest_weights) checks all 2^31 pairs, and the
+// GPU test suite runs it.  (One Newton step, which makes y = RN32(1 / b)
+// for every gap, cost two FMAs per frame and 3 % of the frame-cache launch.)
+// A gap of 0 (never used for a weight) gives a non-finite value.
+__device__ __forceinline__ float sweep_recip(float b) { return __builtin_amdgcn_rcpf(b); }
 
 typedef __attribute__((address_space(3))) const uint8_t lds_byte;
 
@@ -556,6 +555,9 @@ __device__ __forceinline__ void sweep_start(lds_byte* L, uint32_t off, uint32_t 
     s.y = sweep_recip(s.gap);
 }
 
+// Cache-policy bits of a buffer store on gfx950: NT (non-temporal).
+constexpr int STORE_NT = 2;
+
 // Raw buffer descriptor of one frame of the output (base, size in bytes):
 // stores address it with a scalar base and a 32-bit lane offset, and stores
 // at or beyond `bytes` are dropped by the hardware range check.
@@ -580,64 +582,100 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(char* base, uint32_
 // at most KB + 7 bytes past the window's staged records: record k + 2 <=
 // last + 2 starts at most KB bytes past the control's end, which is inside
 // the window, and the read covers two dwords from there.
+// Record k + 2 of the sweep decoded with its frame as a float (frames are
+// integers below 2^16, exact in float).  The alignbyte selector uses the
+// offset's low two bits only, so the offset is passed as it is.
+template <uint32_t KB, bool NANFIX>
+__device__ __forceinline__ void key_decode_f(const RawKey& r, uint32_t off, float& f, float& v)
+{
+    if (KB == 8u) {
+        f = (float)(r.lo & 0xFFFFu);
+        v = __uint_as_float(r.hi);
+    } else {
+        const uint32_t x = __builtin_amdgcn_alignbyte(r.hi, r.lo, off); // bytes off .. off+3
+        f = (float)(x & 0xFFu);
+        v = sweep_half<NANFIX>((x >> 8) & 0xFFFFu);
+    }
+}
+
+// Frames tBegin .. tBegin+cn-1 (all inside [0, F)) of one control from state
+// s at frame tBegin; its records are in LDS at byte offset off (record i at
+// off + i * KB).  Frame i goes to out + i * strideBytes + dstBytes, with out
+// the same for every lane: a buffer store with the frame's base in scalar
+// registers and the lane's 32-bit offset (strideBytes < 2^32, dstBytes <
+// strideBytes; the host checks the frame size).
+// The state update is branch-free.  Key frames are carried as floats (exact
+// integers; the frame counter is a wave-uniform float), so the weight's
+// numerator and the key gap are single float subtractions.  The key counter
+// is replaced by the byte offset pP of record k + 2 (the record read ahead):
+// k < last exactly when pP < e2 = off + (last + 2) * KB.  Record k + 2 is
+// read every step and selected on advance.  Step i issues that LDS read
+// first and produces frame i - 1's output while the read is in flight, then
+// updates the state for frame i.  Once k + 2 passes the last record the read
+// returns whatever follows it in LDS (the next control's records or
+// padding); such a record only becomes key k + 1 when k reaches the last
+// key, where the output is vk and never uses it.  The read reaches at most
+// KB + 7 bytes past the window's staged records: record k + 2 <= last + 2
+// starts at most KB bytes past the control's end, which is inside the
+// window, and the read covers two dwords from there.
 template <uint32_t KB, bool NANFIX>
 __device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t last, const SweepState& s,
                                              int tBegin, int cn, char* out, uint32_t dstBytes, uint64_t strideBytes)
 {
-    uint32_t k = s.k;
-    int fk = s.fk, fn = s.fn;
+    const float INF = __builtin_inff();
+    float fk = (float)s.fk, fn = (float)s.fn;
     float vk = s.vk, vn = s.vn;
     float gap = s.gap, y = s.y;
-    // byte offset of record k + 2; the record is carried raw (the LDS read in
-    // flight) and decoded where the next step needs it
-    uint32_t pOff = off + (k + 2u) * KB;
-    RawKey raw = lds_fetch(L, pOff);
-    // step i's values for frame i's output, produced in step i + 1; at the
-    // last key fkS is set past every frame, so the one comparison
-    // fkS >= frame selects vk both before the first key and after the last
-    // (a select, where `fk >= frame || k == last` compiled to two branches)
-    int fkS = 0;
-    float vkS = 0.0f, vnS = 0.0f;
-    float gapS = 1.0f, yS = 1.0f;
-    auto update = [&](int frame) {
-        int fp;
-        float vp;
-        key_decode<KB, NANFIX>(raw, fp, vp);
-        const bool adv = (k < last) & (fn <= frame);
-        const uint32_t step = adv ? 1u : 0u;
-        const float gNew = (float)(fp - fn);
+    uint32_t pP = off + (s.k + 2u) * KB;
+    const uint32_t e2 = off + (last + 2u) * KB;
+    bool more = s.k < last; // key k + 1 exists
+    RawKey raw = lds_fetch(L, pP);
+    // the output's key frame: past every frame at the last key, so the one
+    // comparison fkS >= frame selects vk both before the first key and after
+    // the last
+    float fkS = more ? fk : INF;
+    float frame = (float)tBegin;
+    auto update = [&]() {
+        float fp, vp;
+        // the record read a step ago is consumed here on every lane: left to
+        // itself the compiler re-reads it on advance only, behind a branch
+        // that waits for the read
+        asm volatile("" ::"v"(raw.lo), "v"(raw.hi));
+        key_decode_f<KB, NANFIX>(raw, pP, fp, vp);
+        asm volatile("" : "+v"(fp), "+v"(vp));
+        const bool adv = more & (fn <= frame);
+        const float gNew = fp - fn;
         const float yNew = sweep_recip(gNew);
         fk = adv ? fn : fk;
         vk = adv ? vn : vk;
         fn = adv ? fp : fn;
         vn = adv ? vp : vn;
-        k += step;
         gap = adv ? gNew : gap;
         y = adv ? yNew : y;
-        pOff += step * KB;
-        raw = lds_fetch(L, pOff);
-        fkS = k == last ? 0x7fffffff : fk;
-        vkS = vk;
-        vnS = vn;
-        gapS = gap;
-        yS = y;
+        pP += adv ? KB : 0u;
+        raw = lds_fetch(L, pP);
+        more = pP < e2;
+        fkS = more ? fk : INF;
     };
-    auto output = [&](int frame) {
-        const float alpha = sweep_alpha(frame - fkS, gapS, yS);
-        const float lerp = vkS + alpha * (vnS - vkS);
-        const float val = fkS >= frame ? vkS : lerp;
+    auto output = [&]() {
+        const float alpha = sweep_alpha(frame - fkS, gap, y); // frame - fkS: exact (integers < 2^16)
+        const float lerp = vk + alpha * (vn - vk);
+        const float val = fkS >= frame ? vk : lerp;
+        // non-temporal store (the cache is written once here and read by
+        // later launches): 8 % faster on smoke256 than a default store
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(val), frame_rsrc(out, (uint32_t)strideBytes),
-                                              (int)dstBytes, 0, 0);
+                                              (int)dstBytes, 0, STORE_NT);
         out += strideBytes;
     };
     // one basic block per step: the record read at the end of a step stays
     // there, and the next step's output stage runs while it is in flight
-    update(tBegin);
+    update();
     for (int i = 1; i < cn; ++i) {
-        output(tBegin + i - 1);
-        update(tBegin + i);
+        output();
+        frame += 1.0f;
+        update();
     }
-    output(tBegin + cn - 1);
+    output();
 }
 
 template <uint32_t NW>
