@@ -7,10 +7,12 @@ empty leaves), VBTPACK4-encoded by this repository's encoder, resident in
 HBM.  One step = decode every frame (t = 0..119) of every active leaf into the
 packed compact active-leaf cache (meaningful controls only).  Also reported:
 point-query throughput/latency of the batched kernel vs the plain
-one-request-per-launch GPU decode, the serving loop (five more resident assets:
-the three fire fields, the liquid level set and the scientific field; Poisson
-and saturation traces against one-request-per-launch decode), and the CPU
-oracle's decode rate on the host cores (bounded sample).
+one-request-per-launch GPU decode (smoke256 and the scientific field), the
+CPU-runnable smoke64 configuration, the serving loop (five more resident
+assets: the three fire fields, the liquid level set and the scientific field;
+Poisson and saturation traces against one-request-per-launch decode), and the
+reference's own CPU decoders (compiled from its sources) and the oracle on
+the host cores (bounded samples, core count and CPU model recorded).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -56,72 +58,182 @@ def measured_traffic(workload: str, kernel: str):
     return None if rec is None else rec.get("traffic_bytes")
 
 
-def cpu_baseline(data: bytes, frames: int, budget_s: float, threads: int) -> dict:
-    """Oracle (C restatement of the reference decode, -O2) on the host cores:
-    packed frame-cache decode of whole frames, one frame per thread."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_py
+def host_cpu() -> dict:
+    """Host cores this run may use (the process's CPU affinity, capped by
+    OMP_NUM_THREADS when set) and the CPU model."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": nproc, "affinity_cores": aff, "omp_num_threads": omp or None, "threads": threads,
+            "model": model}
 
-    import ctypes
 
-    O = oracle_py.oracle()
-    of = oracle_py.OracleFile(data)
-    _, offs = of.cache_packed(0)  # the packed-cache offsets (computed once)
-    per_frame_bytes = 4.0 * float(offs[-1])
-
-    def decode(f, t, out):
-        rc = O.vbto_frame_cache_packed(ctypes.byref(f.vf), t, offs.ctypes.data, out.ctypes.data)
-        if rc != 0:
-            raise RuntimeError(oracle_py.last_error())
-
-    # one thread first (the reference's serial loop)
-    buf = np.empty(int(offs[-1]), np.float32)
-    t0 = time.perf_counter()
-    n1 = 0
-    while time.perf_counter() - t0 < budget_s / 3:
-        decode(of, n1 % frames, buf)
-        n1 += 1
-    dt1 = time.perf_counter() - t0
-    # all threads (independent frames; ctypes releases the GIL)
+def _rate(fn, seconds: float, threads: int) -> tuple:
+    """(calls, elapsed s) of fn(thread index, call index) run on `threads`
+    host threads until `seconds` have passed (the decoders release the GIL)."""
     counter = {"n": 0}
     lock = threading.Lock()
-    stop = time.perf_counter() + budget_s * 2 / 3
+    stop = time.perf_counter() + seconds
 
     def work(tid):
-        f = oracle_py.OracleFile(data)
-        b = np.empty(int(offs[-1]), np.float32)
-        t = tid
+        i = tid
         while time.perf_counter() < stop:
-            decode(f, t % frames, b)
+            fn(tid, i)
             with lock:
                 counter["n"] += 1
-            t += threads
+            i += threads
 
-    ts = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
     t0 = time.perf_counter()
     for th in ts:
         th.start()
     for th in ts:
         th.join()
-    dtn = time.perf_counter() - t0
-    return {
-        "value": round(counter["n"] * per_frame_bytes / dtn / 1e9, 4),
-        "unit": "GB/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{counter['n']} whole-frame cache decodes ({dtn:.1f} s, {threads} threads) of the same asset; "
-                  f"1 thread: {n1} frames in {dt1:.1f} s = {n1 * per_frame_bytes / dt1 / 1e9:.4f} GB/s",
-        "single_thread_gbs": round(n1 * per_frame_bytes / dt1 / 1e9, 4),
-    }
+    return counter["n"], time.perf_counter() - t0
+
+
+def cpu_baseline(data: bytes, frames: int, budget_s: float, threads: int) -> dict:
+    """The packed frame-cache decode of whole frames on the host cores, one
+    frame per thread: the reference's own per-value decoders (compiled from
+    its sources into oracle/_ref) when present, and the oracle (the C
+    restatement, -O3) beside it; one thread (the reference's serial frame
+    loop) and all threads (the reference's OpenMP-over-work-items pattern)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+
+    of = oracle_py.OracleFile(data)
+    offs = of.packed_offsets()
+    per_frame_bytes = 4.0 * float(offs[-1])
+    decoders = {}
+    if oracle_py.have_ref():
+        rf = oracle_py.RefFile(data)
+        decoders["reference"] = lambda t, out: rf.cache_frame_into(t, offs, out)
+    import ctypes
+
+    O = oracle_py.oracle()
+
+    def port(t, out):
+        if O.vbto_frame_cache_packed(ctypes.byref(of.vf), t, offs.ctypes.data, out.ctypes.data) != 0:
+            raise RuntimeError(oracle_py.last_error())
+
+    decoders["port"] = port
+    share = budget_s / len(decoders)
+    res = {}
+    for kind, dec in decoders.items():
+        bufs = [np.empty(int(offs[-1]), np.float32) for _ in range(threads)]
+        n1, dt1 = _rate(lambda k, i: dec(i % frames, bufs[k]), share / 3, 1)
+        nn, dtn = _rate(lambda k, i: dec(i % frames, bufs[k]), share * 2 / 3, threads)
+        res[kind] = {"gbs": round(nn * per_frame_bytes / dtn / 1e9, 4),
+                     "single_thread_gbs": round(n1 * per_frame_bytes / dt1 / 1e9, 4),
+                     "sample": f"{nn} whole-frame cache decodes ({dtn:.1f} s, {threads} threads) of the same asset; "
+                               f"1 thread: {n1} frames in {dt1:.1f} s"}
+    head = "reference" if "reference" in res else "port"
+    return {"value": res[head]["gbs"], "unit": "GB/s", "cores": threads, "kind": head,
+            "sample": res[head]["sample"], "single_thread_gbs": res[head]["single_thread_gbs"],
+            "by_kind": res, "host": host_cpu()}
+
+
+def random_queries(dims, n, seed=0x5EED0001) -> np.ndarray:
+    W, H, D, F = dims
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.integers(0, W, n), rng.integers(0, H, n), rng.integers(0, D, n),
+                     rng.integers(0, F, n)], axis=1).astype(np.uint32)
+
+
+def cpu_query_rates(data: bytes, dims, threads: int, seconds: float, scientific: bool) -> dict:
+    """Point queries on the host cores over the same seeded random queries
+    as the GPU figures: the oracle on 1 and all threads, and for scientific
+    assets the reference's serial query loop (decodeScientificValueAtCpu, the
+    reference's own CPU query harness) when oracle/_ref is present."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+
+    of = oracle_py.OracleFile(data)
+    q = random_queries(dims, 65536)
+    out = {}
+    for name, th in (("port_1_thread", 1), (f"port_{threads}_threads", threads)):
+        n, dt = _rate(lambda k, i: of.query(q, threads=th, check=False), seconds / 3, 1)
+        out[name] = round(n * len(q) / dt, 1)
+    if scientific and oracle_py.have_ref():
+        rf = oracle_py.RefFile(data)
+        n, dt = _rate(lambda k, i: rf.sci_values(q[:16384]), seconds / 3, 1)
+        out["reference_1_thread"] = round(n * 16384 / dt, 1)
+    out["unit"] = "queries/s"
+    out["queries"] = "65,536 uniform (x, y, z, t), seed 0x5EED0001"
+    return out
+
+
+def cfg1_bench(hip, cpu: bool) -> dict:
+    """The CPU-runnable configuration: the reference-encoded 64^3 x 32 smoke
+    asset (tests/golden/smoke64), decode of the full frame t = 17 plus 10,000
+    seeded uniform (x, y, z, t) point queries, on the GPU (HIP events) and on
+    one host thread (oracle; the reference's per-value decode of frame 17's
+    controls beside it)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import golden_io
+    import oracle_py
+
+    data = golden_io.vbtp("smoke64")
+    a = hip.Asset(data)
+    s = a.stream
+    W, H, D, F = a.dims
+    q = random_queries(a.dims, 10000)
+    dq = hip.DeviceArray.from_host(q, s)
+    dout = hip.DeviceArray(len(q), np.float32, s)
+    dense = hip.DeviceArray((D, H, W), np.float32, s)
+
+    def gpu_step():
+        a.dense_into(dense, 17)
+        a.query_into(dq, len(q), dout)
+
+    gpu_ms = timed(hip, s, gpu_step, 50, 3)
+    res = {"workload": "smoke64 (reference-encoded) 64x64x64 x 32: dense frame 17 + 10,000 seeded point queries",
+           "gpu_ms": round(gpu_ms, 4)}
+    if not cpu:
+        a.close()
+        return res
+    of = oracle_py.OracleFile(data)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        of.frame(17)
+        of.query(q)
+        reps += 1
+        if time.perf_counter() - t0 > 1.0:
+            break
+    cpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    res.update({"cpu_port_1_thread_ms": round(cpu_ms, 3), "gpu_over_cpu": round(cpu_ms / gpu_ms, 1)})
+    if oracle_py.have_ref():
+        rf = oracle_py.RefFile(data)
+        offs = of.packed_offsets()
+        buf = np.empty(int(offs[-1]), np.float32)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 0.5:
+            rf.cache_frame_into(17, offs, buf)
+            k += 1
+        res["cpu_reference_frame17_controls_1_thread_ms"] = round((time.perf_counter() - t0) / k * 1e3, 3)
+    a.close()
+    return res
 
 
 def query_bench(asset, hip, n_batch: int, reps: int) -> dict:
     """Batched point queries (one launch over the batch, 8 lanes per query)
     vs the plain GPU decode (one thread per query); latency per batch."""
-    W, H, D, F = asset.dims
-    rng = np.random.default_rng(0x5EED0001)
-    q = np.stack([rng.integers(0, W, n_batch), rng.integers(0, H, n_batch), rng.integers(0, D, n_batch),
-                  rng.integers(0, F, n_batch)], axis=1).astype(np.uint32)
+    q = random_queries(asset.dims, n_batch)
     s = asset.stream
     dq = hip.DeviceArray.from_host(q, s)
     out = hip.DeviceArray(n_batch, np.float32, s)
@@ -460,8 +572,9 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     w = synth.WORKLOADS[args.workload]
+    HT = host_cpu()["threads"]
     t0 = time.time()
-    data = w.encode(threads=min(16, os.cpu_count() or 1))
+    data = w.encode(threads=HT)
     enc_s = time.time() - t0
     print(f"[bench] encoded {args.workload} {w.dims}: {len(data) / 1e6:.1f} MB in {enc_s:.1f} s", file=sys.stderr,
           flush=True)
@@ -583,9 +696,22 @@ def main():
     for n in want:
         if n not in extra:
             t0 = time.time()
-            raws[n] = synth.WORKLOADS[n].encode(threads=min(16, os.cpu_count() or 1))
+            raws[n] = synth.WORKLOADS[n].encode(threads=HT)
             extra[n] = hip.Asset(raws[n])
             print(f"[bench] asset {n}: encoded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    if not args.no_queries:
+        if not args.no_cpu:
+            line["queries"]["cpu"] = cpu_query_rates(data, asset.dims, HT, 6.0, False)
+        if "sci192" in extra:
+            sa = extra["sci192"]
+            sq = {"random_65536": query_bench(sa, hip, 65536, 50), "random_256": query_bench(sa, hip, 256, 200)}
+            if not args.no_cpu:
+                sq["cpu"] = cpu_query_rates(raws["sci192"], sa.dims, HT, 6.0, True)
+            line["queries_sci192"] = sq
+        line["cfg1_smoke64"] = cfg1_bench(hip, not args.no_cpu)
+        c1 = line["cfg1_smoke64"]
+        print(f"[bench] cfg1 smoke64: GPU {c1['gpu_ms']} ms, CPU {c1.get('cpu_port_1_thread_ms')} ms (1 thread)",
+              file=sys.stderr, flush=True)
     if not args.no_batch:
         line["batch_single"] = batch_bench(hip, extra, [args.workload], args.steps, args.batch_tune_reps)
         names = [x for x in args.batch_assets.split(",") if x]
@@ -596,10 +722,10 @@ def main():
               f"{bb['shape']}; single {line['batch_single']['batched_graph']['ms']} ms vs one-shot "
               f"{line['batch_single']['plain_per_asset']['ms']} ms", file=sys.stderr, flush=True)
     if not args.no_march:
-        line["march"] = march_bench(hip, extra, raws, args.steps, min(16, os.cpu_count() or 1),
+        line["march"] = march_bench(hip, extra, raws, args.steps, HT,
                                     0.0 if args.no_cpu else 3.0)
     if not args.no_cycles:
-        line["cycles"] = cycles_bench(hip, extra, raws, args.steps, min(16, os.cpu_count() or 1),
+        line["cycles"] = cycles_bench(hip, extra, raws, args.steps, HT,
                                       0.0 if args.no_cpu else 2.0)
     if not args.no_serve:
         serve = {n: extra[n] for n in [args.workload] + [x for x in args.serve_assets.split(",") if x]}
@@ -612,7 +738,7 @@ def main():
         if a is not asset:
             a.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(data, F, args.cpu_seconds, min(16, os.cpu_count() or 1))
+        line["cpu_baseline"] = cpu_baseline(data, F, args.cpu_seconds, HT)
     asset.close()
     print(json.dumps(line), flush=True)
 

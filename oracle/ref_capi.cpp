@@ -1,9 +1,10 @@
 // ref_capi.cpp -- C-ABI wrapper over the reference's own decode sources, built
 // only into oracle/_ref/libvbtref.so (test infrastructure, never shipped).
 //
-// This file contains no decode logic: every entry point forwards to the
-// reference function named in its comment, catches its exceptions and returns
-// the message.  It lets the tests check the oracle restatement against the
+// This file contains no decode logic of its own: every entry point forwards
+// to the reference functions named in its comment (one loops over leaves and
+// controls the way the reference's CPU frame decode does), catches their
+// exceptions and returns the message.  It lets the tests check the oracle restatement against the
 // real reference implementation on the same bytes.
 #include "render_temporal_decode.h"
 #include "render_temporal_payload.h"
@@ -131,6 +132,24 @@ int vbtref_sci_value(void* h, uint32_t x, uint32_t y, uint32_t z, uint32_t t, fl
     }
 }
 
+// render/src/scientific_decode.cpp:335 over a batch of (x, y, z, t) queries,
+// one after the other (the serial query loop of
+// render/src/cpu_query_bench_main.cpp:161-170)
+int vbtref_sci_values(void* h, const uint32_t* q, uint64_t n, float* out)
+{
+    try {
+        const auto& f = *static_cast<vbt::render::VbtFile*>(h);
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t* p = q + 4u * i;
+            vbt::render::Query4D qq{p[0], p[1], p[2], p[3], vbt::render::leafIndexForVoxel(f.header, p[0], p[1], p[2])};
+            out[i] = vbt::render::decodeScientificValueAtCpu(f, qq);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 // render/src/scientific_decode.cpp:361
 int vbtref_sci_frame(void* h, uint32_t t, float* out)
 {
@@ -138,6 +157,51 @@ int vbtref_sci_frame(void* h, uint32_t t, float* out)
         const auto& f = *static_cast<vbt::render::VbtFile*>(h);
         const auto frame = vbt::render::reconstructScientificFrameCpu(f, t);
         std::memcpy(out, frame.data(), frame.size() * sizeof(float));
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// The packed frame cache of frame t through the reference's per-value
+// decoders, in the order of the reference's CPU frame decode
+// (render/src/render_smoke_vbt_bench_main.cpp:286-330: parse each leaf once,
+// then decodeRenderTemporalLeafControlValue for every coarse and fine
+// control; Mode2 leaves: decodeRenderTemporalLeafShellVoxelValue for every
+// occupied voxel, which in voxel order are the shell stream's controls).
+// Active leaves (header mode != EMPTY) are written at ctrlOffset[rank].
+int vbtref_cache_frame(void* h, int t, const uint64_t* ctrlOffset, float* out)
+{
+    try {
+        const auto& f = *static_cast<vbt::render::VbtFile*>(h);
+        const uint8_t* payload = reinterpret_cast<const uint8_t*>(f.payloadWords.data());
+        const int frames = static_cast<int>(f.header.frames);
+        uint32_t rank = 0;
+        for (uint32_t l = 0; l < f.header.leafCount; ++l) {
+            if ((vbt::render::readPackedHeaderWord(f, l) & 3u) == 0u) continue;
+            const uint32_t wb = f.offsetsWords[l], we = f.offsetsWords[l + 1u];
+            const uint8_t* leaf = payload + static_cast<size_t>(wb) * 4u;
+            const auto v = vbt::parseRenderTemporalPackedLeaf(leaf, static_cast<size_t>(we - wb) * 4u, true);
+            float* o = out + ctrlOffset[rank++];
+            const int cr = v.header.coarseResolution;
+            const uint16_t cc = static_cast<uint16_t>(cr * cr * cr);
+            for (uint16_t i = 0; i < cc; ++i)
+                *o++ = vbt::decodeRenderTemporalLeafControlValue(leaf, v.header, v.prefix, v.layout, false, true,
+                                                                 frames, i, t);
+            if (v.header.mode == vbt::TemporalFirstPackedMode::TEMPORAL_FINE6) {
+                const int fr = v.header.fineResolution;
+                const uint16_t fc = static_cast<uint16_t>(fr * fr * fr);
+                for (uint16_t i = 0; i < fc; ++i)
+                    *o++ = vbt::decodeRenderTemporalLeafControlValue(leaf, v.header, v.prefix, v.layout, true, true,
+                                                                     frames, i, t);
+            } else if (v.header.mode == vbt::TemporalFirstPackedMode::TEMPORAL_FINE_COMPACT) {
+                const auto occ = vbt::unpackRenderTemporalShellOccupancySection(leaf + v.layout.fine.descriptorOffset,
+                                                                                v.layout.fine.descriptorBytes);
+                for (uint16_t vox = 0; vox < 512; ++vox)
+                    if ((occ.shellMask[vox >> 6] >> (vox & 63u)) & 1u)
+                        *o++ = vbt::decodeRenderTemporalLeafShellVoxelValue(leaf, v, true, frames, vox, t);
+            }
+        }
         return 0;
     } catch (const std::exception& e) {
         return fail(e);

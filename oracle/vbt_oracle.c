@@ -31,6 +31,15 @@
  *                                reproduced at every voxel; an independent
  *                                float64 numpy evaluation (searchsorted-based)
  *                                agrees within 1e-6.
+ *   composition (Fine6, Mode2,   the reference encoder's own sampled RMSE of
+ *   sign rules)                  the leaves it saved (smoke64: Grid4 + Fine6,
+ *                                liquid_mini: Grid4 + Mode2) reproduced to its
+ *                                9 printed decimals; a Fine6 leaf of linear
+ *                                grids decodes to their closed-form sum; a
+ *                                Mode2 leaf follows a truth table of the three
+ *                                sign rules written from the reference's
+ *                                conditions, bit for bit
+ *                                (tests/test_composition_pins.py).
  *   page table                   invariant: rank among active leaves.
  *   scientific DCT + residuals   oracle/_ref (reference scientific_decode.cpp
  *                                compiled here) on golden scientific assets
@@ -38,7 +47,7 @@
  *                                both dense subtypes; closed-form DCT-II
  *                                inverse in float64 numpy.
  *
- * Build: gcc -std=c11 -O2 -ffp-contract=off (see oracle/Makefile).
+ * Build: gcc -std=c11 -O3 -ffp-contract=off (see oracle/Makefile).
  */
 #include "vbt_oracle.h"
 

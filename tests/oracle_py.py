@@ -151,6 +151,8 @@ def ref():
         L.vbtref_free_file.argtypes = [ctypes.c_void_p]
         L.vbtref_sci_value.argtypes = [ctypes.c_void_p] + [ctypes.c_uint32] * 4 + [ctypes.POINTER(ctypes.c_float)]
         L.vbtref_sci_frame.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.vbtref_cache_frame.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.vbtref_sci_values.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
         _R = L
     return _R
 
@@ -325,3 +327,41 @@ def cycles_sample(blob: np.ndarray, xyz: np.ndarray, flags: int = 0, threads: in
                             threads) != 0:
         raise ValueError(L.vbto_cycles_error().decode())
     return out
+
+
+class RefFile:
+    """A VBTPACK4 file loaded by the reference's own loader (oracle/_ref/libvbtref.so)."""
+
+    def __init__(self, data: bytes):
+        import tempfile
+
+        with tempfile.NamedTemporaryFile(suffix=".vbtp", delete=False) as fh:
+            fh.write(data)
+            path = fh.name
+        try:
+            self.h = ref().vbtref_load_file(path.encode())
+        finally:
+            os.unlink(path)
+        if not self.h:
+            raise ValueError(ref().vbtref_last_error().decode())
+
+    def cache_frame_into(self, t: int, offs: np.ndarray, out: np.ndarray):
+        """The packed cache of frame t through the reference's per-value decoders."""
+        if ref().vbtref_cache_frame(self.h, int(t), offs.ctypes.data, out.ctypes.data) != 0:
+            raise ValueError(ref().vbtref_last_error().decode())
+
+    def sci_values(self, q: np.ndarray) -> np.ndarray:
+        """decodeScientificValueAtCpu for each (x, y, z, t) row, serially."""
+        q = np.ascontiguousarray(q, dtype=np.uint32)
+        out = np.empty(len(q), np.float32)
+        if ref().vbtref_sci_values(self.h, q.ctypes.data, len(q), out.ctypes.data) != 0:
+            raise ValueError(ref().vbtref_last_error().decode())
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            ref().vbtref_free_file(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -488,3 +488,38 @@ def test_cache_window_passes_stay_in_bounds(block, window):
             assert (lo[lanes] >= w0 * 4).all() and (hi[lanes] <= (w0 + nw) * 4).all()
             served[lanes] += 1
         np.testing.assert_array_equal(served, pending.astype(int))
+
+
+@need_ref
+@pytest.mark.parametrize("name", ["smoke64", "liquid_mini"])
+def test_packed_cache_equals_the_reference_per_value_decode(name):
+    """Every slot of the oracle's packed frame cache against the reference's
+    per-value decoders (oracle/_ref: decodeRenderTemporalLeafControlValue and
+    decodeRenderTemporalLeafShellVoxelValue over every active leaf), bit for
+    bit, at every frame."""
+    data = golden_io.vbtp(name)
+    of = oracle_py.OracleFile(data)
+    rf = oracle_py.RefFile(data)
+    offs = of.packed_offsets()
+    got = np.zeros(int(offs[-1]), np.float32)
+    for t in range(of.header.frames):
+        rf.cache_frame_into(t, offs, got)
+        exp = of.cache_packed(t)[0]
+        assert (got.view(np.uint32) == exp.view(np.uint32)).all(), t
+    rf.close()
+
+
+@need_ref
+@pytest.mark.parametrize("name", ["sci_mini", "sci_dense_mini"])
+def test_scientific_queries_equal_the_reference_cpu_query_loop(name):
+    """The oracle's scientific point queries against the reference's
+    decodeScientificValueAtCpu in its serial query loop, bit for bit."""
+    data = golden_io.vbtp(name)
+    of = oracle_py.OracleFile(data)
+    rf = oracle_py.RefFile(data)
+    W, H, D, F = of.dims
+    rng = np.random.default_rng(0x5EED0001)
+    q = np.stack([rng.integers(0, W, 5000), rng.integers(0, H, 5000), rng.integers(0, D, 5000),
+                  rng.integers(0, F, 5000)], axis=1).astype(np.uint32)
+    assert (rf.sci_values(q).view(np.uint32) == of.query(q).view(np.uint32)).all()
+    rf.close()
