@@ -338,6 +338,11 @@ def serving_bench(hip, loadgen, assets: dict, requests: int, seed: int = 0x5EED0
                 "cache_p50_us": round(r["cp50Us"], 1), "cache_p99_us": round(r["cp99Us"], 1),
                 "failed": r["failed"]}
 
+    # both paths warmed up on a short trace of their own first (kernel loads,
+    # the device loop's first launch, first touches of the outputs)
+    loadgen.run(handles, loadgen.MODE_PLAIN, 0.0, 300, cf, 65536, seed + 7)
+    loadgen.run(handles, loadgen.MODE_SERVER, 0.0, 300, cf, 65536, seed + 7, srv, ids)
+    srv.reset_stats()
     plain_sat = loadgen.run(handles, loadgen.MODE_PLAIN, 0.0, requests, cf, 65536, seed)
     batched_sat = loadgen.run(handles, loadgen.MODE_PER_LAUNCH_BATCHED, 0.0, requests, cf, 65536, seed)
     server_sat = loadgen.run(handles, loadgen.MODE_SERVER, 0.0, requests, cf, 65536, seed, srv, ids)

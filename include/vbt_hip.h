@@ -392,12 +392,15 @@ typedef struct {
     uint64_t stagingBytes;   /* pinned host staging for query inputs (default 256 MiB) */
     uint32_t maxInflight;    /* admitted, unfinished requests at once (default 4096) */
     uint32_t blocksPerCU;    /* persistent workgroups per CU, 256 lanes each (default 2) */
-    uint32_t queryItem;      /* point queries per work item, multiple of 32 (default 128) */
+    uint32_t queryItem;      /* point queries per work item, multiple of 32 (default 256) */
     uint32_t cacheItem;      /* packed-cache slots per work item, multiple of 256 (default 1024) */
     uint32_t idleUs;         /* device loop exits after this long without new requests (default 2000) */
     uint32_t leaseMs;        /* device loop exits (and is relaunched) at least this often (default 250) */
     uint32_t frameBuildsInLoop; /* 0 (default): whole-frame builds run as one-shot frame-cache kernels on a
                                    stream beside the device loop; 1: as cacheItem-slot work items of the loop */
+    uint32_t frameIndex;        /* 1 (default): registering a render asset builds a frame index owned by the
+                                   server (its bytes count as resident), and one-shot frame builds read it
+                                   (same values); 0: frame builds search the keyframe pool */
 } vbt_server_config;
 
 typedef struct {
@@ -495,17 +498,20 @@ int vbt_march_rays(vbt_asset a, const float* d_cache576, const vbt_march_params*
                    float* d_out, void* stream);
 
 /* Frame index for repeated one- and two-frame packed-cache builds (serving,
- * scrubbing): each slot's keyframe records regrouped by eighth of the frame
- * range, so such a build reads short adjacent runs instead of one keyframe
- * line per slot.  No counterpart in the reference (its decoder reads the
- * payload per value); it changes no result: with the index, frame builds of
- * one or two frames in the packed layout read it, other builds and leaves
- * whose keys do not strictly increase keep the bracket search.  Costs
- * 4 * 8 * (controlsPerFrame + 1) bytes of offsets plus 8 bytes per indexed
- * record (*bytes); synchronizes the stream.  Render assets only.  Build and
- * drop change the asset: call them while no other call on it runs. */
+ * scrubbing): each slot's keyframe records regrouped by bin of the frame range
+ * (min(32, frames) bins), so such a build reads short adjacent runs instead
+ * of one keyframe line per slot.  No counterpart in the reference (its
+ * decoder reads the payload per value); it changes no result: with the index,
+ * frame builds of one or two frames in the packed layout read it, other
+ * builds and leaves whose keys do not strictly increase keep the bracket
+ * search.  Costs 4 * bins * (controlsPerFrame + 1) bytes of offsets plus 4
+ * bytes per indexed record (8 when a run holds fp32 records) (*bytes, also
+ * added to the asset's deviceBytes); synchronizes the stream.  Render assets
+ * only.  Build and drop change the asset: call them while no other call on
+ * it runs. */
 int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream);
-/* Frees the frame index (waits for the device first). */
+/* Frees the frame index, after waiting for the last build that read it on
+ * each stream. */
 int vbt_asset_drop_frame_index(vbt_asset a);
 
 /* ---- Cycles resident blob (cycles_native_vbt/, blender_patch/native_cycles_vbt.patch) ----

@@ -97,13 +97,25 @@ def a_cache_no_check(a, t):
     return out.host()[0]
 
 
-def test_full_size_smoke256_sampled_frames():
-    data = synth.WORKLOADS["smoke256"].encode(threads=16)
+@pytest.mark.parametrize("name", ["smoke256", "liquid320"])
+def test_full_size_every_frame_against_the_oracle(name):
+    """Every frame of the full-size asset built one at a time from the index
+    (and in pairs), bit for bit against the oracle and the builds without it."""
+    from workloads import THREADS, encoded
+
+    data = encoded(name)
     a = hip.Asset(data)
+    o = oracle_py.OracleFile(data)
     F = a.dims[3]
-    frames = [0, 1, 14, 15, 16, 59, 60, 104, 105, F - 1]
-    plain = {t: a.frame_cache(t, 1)[0] for t in frames}
-    a.build_frame_index()
-    for t in frames:
-        assert np.array_equal(bits(a.frame_cache(t, 1)[0]), bits(plain[t])), t
+    frames = list(range(F)) if name == "smoke256" else list(range(0, F, 7)) + [F - 1]
+    exp = o.cache_packed_frames(frames, threads=THREADS)
+    plain = {t: a.frame_cache(t, 1)[0] for t in frames[::5]}
+    assert a.build_frame_index() > 0
+    for i, t in enumerate(frames):
+        got = a.frame_cache(t, 1)[0]
+        assert np.array_equal(bits(got), bits(exp[i])), (name, t)
+        if t in plain:
+            assert np.array_equal(bits(got), bits(plain[t])), (name, t)
+    two = a.frame_cache(F - 2, 2)
+    assert np.array_equal(bits(two[1]), bits(exp[-1])), name
     a.close()

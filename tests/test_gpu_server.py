@@ -64,8 +64,9 @@ def submit(srv, ids, req):
     return srv.submit_query(ids[name], arg) if kind == "q" else srv.submit_cache(ids[name], arg)
 
 
-@pytest.mark.parametrize("in_loop", [0, 1], ids=["frames-beside-loop", "frames-in-loop"])
-def test_alone_equals_mix(assets, in_loop):
+@pytest.mark.parametrize("in_loop,frame_index", [(0, 1), (0, 0), (1, 0)],
+                         ids=["frames-beside-loop-indexed", "frames-beside-loop", "frames-in-loop"])
+def test_alone_equals_mix(assets, in_loop, frame_index):
     rng = np.random.default_rng(2024)
     probes = []
     for n in NAMES:
@@ -83,7 +84,7 @@ def test_alone_equals_mix(assets, in_loop):
             mix.append(("q", n, rand_queries(rng, dims, max(1, size))))
     ref = {i: expected(assets, r) for i, r in enumerate(probes + mix)}
 
-    srv = hip.Server(frameBuildsInLoop=in_loop)
+    srv = hip.Server(frameBuildsInLoop=in_loop, frameIndex=frame_index)
     ids = {n: srv.register(assets[n][0]) for n in NAMES}
     # probes alone, one at a time
     alone = []
@@ -157,7 +158,7 @@ def test_hbm_budget_rejections(assets):
     a, _ = assets["smoke64"]
     b, _ = assets["liquid_mini"]
     need = a.info.deviceBytes
-    srv = hip.Server(hbmBudgetBytes=need + 64 * 1024, arenaBytes=64 * 1024)
+    srv = hip.Server(hbmBudgetBytes=need + 64 * 1024, arenaBytes=64 * 1024, frameIndex=0)
     i = srv.register(a)
     with pytest.raises(hip.VbtError) as e:
         srv.register(b)
@@ -303,3 +304,28 @@ def test_largest_query_batch_and_item_shapes(assets):
         same(srv.fetch(srv.submit_query(i, q)), exp, f"65536 queries, item {qi}")
         same(srv.fetch(srv.submit_cache(i, 9)), a.frame_cache(9, 1)[0], f"cache, item {ci}")
         srv.close()
+
+
+def test_server_frame_index_counts_as_resident(assets):
+    """With the server's frame index (the default), each registered render
+    asset costs its device bytes plus the index's; scientific assets have
+    none; a budget without room for the index refuses the asset."""
+    a, _ = assets["smoke64"]
+    sci, _ = assets["sci_mini"]
+    probe = hip.Asset(golden_io.vbtp("smoke64"))
+    index_bytes = probe.build_frame_index()  # the same build the server runs
+    probe.close()
+    assert index_bytes > 0
+    srv = hip.Server()
+    srv.register(a)
+    srv.register(sci)
+    assert srv.stats()["hbmInUse"] == a.info.deviceBytes + index_bytes + sci.info.deviceBytes
+    srv.close()
+    tight = hip.Server(hbmBudgetBytes=a.info.deviceBytes + index_bytes // 2, arenaBytes=4096)
+    with pytest.raises(hip.VbtError) as e:
+        tight.register(a)
+    assert e.value.code == 5
+    tight.close()
+    plain = hip.Server(hbmBudgetBytes=a.info.deviceBytes + index_bytes // 2, arenaBytes=4096, frameIndex=0)
+    plain.register(a)
+    plain.close()

@@ -77,7 +77,13 @@ def submit(srv, ids, req):
 @pytest.mark.parametrize("budget", ["default", "tight"])
 def test_full_resident_set_alone_equals_mix(resident, trace, budget):
     probes, mix, ref = trace
-    resident_bytes = sum(a.info.deviceBytes for a, _ in resident.values())
+    # resident bytes as the server counts them (assets and their frame indexes)
+    probe = hip.Server()
+    for n in NAMES:
+        probe.register(resident[n][0])
+    resident_bytes = probe.stats()["hbmInUse"]
+    probe.close()
+    assert resident_bytes > sum(a.info.deviceBytes for a, _ in resident.values())
     cfg = {}
     if budget == "tight":
         # room for about three of the largest frame outputs beside the assets

@@ -400,12 +400,18 @@ __global__ void __launch_bounds__(256) k_cache_exact(AssetDev a, int t0, int nt,
 // instead of one line of the keyframe pool per slot.  Leaves whose keys do not
 // strictly increase, poisoned leaves and empty streams keep no run and take
 // the reference bracket search (cache_slot_exact).
-constexpr uint32_t INDEX_BINS = 8u;
+// Bins of the frame range the index keeps runs for: a one-frame build reads
+// its bin's run of every slot (about keys / bins + 2 records), the index holds
+// about keys + 2 x bins records per slot.  32 bins: liquid320 one-frame builds
+// 99 -> 58 us against 8 bins, for an index of 4.1 x the payload (8 bins:
+// 1.8 x).  At most one bin per frame.
+constexpr uint32_t INDEX_BINS_MAX = 32u;
+__host__ __device__ inline uint32_t index_bins(uint32_t F) { return F < INDEX_BINS_MAX ? (F ? F : 1u) : INDEX_BINS_MAX; }
 
-__device__ __forceinline__ uint32_t index_bin(int t, uint32_t F) { return min(INDEX_BINS - 1u, (uint32_t)t * INDEX_BINS / F); }
+__device__ __forceinline__ uint32_t index_bin(int t, uint32_t F, uint32_t B) { return min(B - 1u, (uint32_t)t * B / F); }
 
 // the run [lo, hi] of records of one slot for bin b; false when the slot keeps none
-__device__ __forceinline__ bool index_run(const AssetDev& a, uint64_t g, uint32_t b, const uint32_t*& W,
+__device__ __forceinline__ bool index_run(const AssetDev& a, uint64_t g, uint32_t b, uint32_t B, const uint32_t*& W,
                                           StreamRef& s, uint32_t& k0, uint32_t& lo, uint32_t& hi)
 {
     uint32_t c = a.groupLeaf[g / CACHE_GROUP];
@@ -413,8 +419,8 @@ __device__ __forceinline__ bool index_run(const AssetDev& a, uint64_t g, uint32_
     const DevLeaf& d = a.leaves[c];
     if ((d.status & STATUS_MASK) != LEAF_OK || !(d.status & FLAG_SWEEP)) return false;
     const uint32_t F = a.F;
-    const uint32_t t0 = (b * F + INDEX_BINS - 1u) / INDEX_BINS;
-    const uint32_t t1 = b + 1u < INDEX_BINS ? ((b + 1u) * F + INDEX_BINS - 1u) / INDEX_BINS - 1u : F - 1u;
+    const uint32_t t0 = (b * F + B - 1u) / B;
+    const uint32_t t1 = b + 1u < B ? ((b + 1u) * F + B - 1u) / B - 1u : F - 1u;
     if (t0 > t1) return false;
     const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
     const bool isCoarse = slot < d.cc;
@@ -454,16 +460,16 @@ __device__ __forceinline__ bool index_run(const AssetDev& a, uint64_t g, uint32_
 
 // counts per bin and slot; *wide is set when a run holds fp32 records (the
 // index then stores 8-byte records, else 4-byte {frame, half bits})
-__global__ void __launch_bounds__(256) k_index_count(AssetDev a, uint32_t* cnt, uint64_t C, uint32_t* wide)
+__global__ void __launch_bounds__(256) k_index_count(AssetDev a, uint32_t* cnt, uint64_t C, uint32_t B, uint32_t* wide)
 {
     const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (g > C) return;
     bool fp32 = false;
-    for (uint32_t b = 0; b < INDEX_BINS; ++b) {
+    for (uint32_t b = 0; b < B; ++b) {
         const uint32_t* W;
         StreamRef s;
         uint32_t k0, lo, hi;
-        const bool run = g < C && index_run(a, g, b, W, s, k0, lo, hi);
+        const bool run = g < C && index_run(a, g, b, B, W, s, k0, lo, hi);
         cnt[b * (C + 1u) + g] = run ? hi - lo + 1u : 0u;
         fp32 = fp32 || (run && s.codec == CODEC_FP32);
     }
@@ -472,15 +478,16 @@ __global__ void __launch_bounds__(256) k_index_count(AssetDev a, uint32_t* cnt, 
 
 // WIDE: {frame, value bits}; else {frame | half bits << 16} (fp16 records only)
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_index_scatter(AssetDev a, const uint32_t* off, void* rec, uint64_t C)
+__global__ void __launch_bounds__(256) k_index_scatter(AssetDev a, const uint32_t* off, void* rec, uint64_t C,
+                                                       uint32_t B)
 {
     const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (g >= C) return;
-    for (uint32_t b = 0; b < INDEX_BINS; ++b) {
+    for (uint32_t b = 0; b < B; ++b) {
         const uint32_t* W;
         StreamRef s;
         uint32_t k0, lo, hi;
-        if (!index_run(a, g, b, W, s, k0, lo, hi)) continue;
+        if (!index_run(a, g, b, B, W, s, k0, lo, hi)) continue;
         const uint32_t at = off[b * (C + 1u) + g];
         const uint32_t kb = keyframe_bytes(s.codec);
         for (uint32_t k = lo; k <= hi; ++k) {
@@ -525,14 +532,14 @@ __global__ void __launch_bounds__(256) k_sum64(const uint32_t* cnt, uint64_t n, 
 // k_cache_exact (for strictly increasing keys the reference bracket is the
 // last key at or before the frame and the next one).
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_t* off, const void* rec, int t0,
-                                                       int nt, float* out, uint64_t total, uint32_t* errFlag)
+__global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_t* off, const void* rec, uint32_t B,
+                                                       int t0, int nt, float* out, uint64_t total, uint32_t* errFlag)
 {
     const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (g >= total) return;
     for (int i = 0; i < nt; ++i) {
         const int t = t0 + i;
-        const uint32_t b = index_bin(t, a.F);
+        const uint32_t b = index_bin(t, a.F, B);
         const uint32_t o0 = off[b * (total + 1u) + g], o1 = off[b * (total + 1u) + g + 1u];
         float val;
         if (o0 == o1) {
@@ -880,10 +887,10 @@ struct vbt_asset_s {
     uint32_t* d_blockLeaf = nullptr; // first active leaf of each 64-slot group of the packed cache
     int cacheTile = CACHE_TILE_DEFAULT;
     Pair* d_blocks = nullptr;
-    uint32_t* d_binOff = nullptr; // frame index: record offsets, INDEX_BINS x (controlsPerFrame + 1)
-    void* d_binRec = nullptr;     // frame index records (binWide: 8-byte {frame, value bits}, else 4-byte)
-    uint64_t binRecords = 0;
-    bool binWide = false;
+    VbtFrameIndex index; // the asset's own frame index (vbt_asset_build_frame_index)
+    // one event per stream that launched a build reading the index, recorded
+    // after the launch: dropping the index waits for them only
+    std::vector<std::pair<hipStream_t, hipEvent_t>> indexUse;
     uint32_t A = 0, bad = 0;
     uint64_t controlsPerFrame = 0, payloadBytes = 0, deviceBytes = 0, fast = 0;
     uint64_t modeCounts[4] = {0, 0, 0, 0};
@@ -927,9 +934,9 @@ template <class T> int dalloc(vbt_asset a, T** p, size_t count)
 void free_asset(vbt_asset a)
 {
     if (!a) return;
-    void* ptrs[] = {a->d_offsets, a->d_payload, a->d_scales, a->d_compact, a->d_status,     a->d_leaves,
-                    a->d_ctrl,    a->d_basis,   a->d_err,    a->d_blocks,  a->d_blockLeaf, a->d_binOff,
-                    a->d_binRec};
+    vbt_asset_drop_frame_index(a);
+    void* ptrs[] = {a->d_offsets, a->d_payload, a->d_scales, a->d_compact, a->d_status,
+                    a->d_leaves,  a->d_ctrl,    a->d_basis,  a->d_err,     a->d_blocks, a->d_blockLeaf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete a;
@@ -957,6 +964,123 @@ ComposeDev to_dev(const vbt_compose* c)
 }
 
 } // namespace
+
+namespace {
+
+void launch_indexed(vbt_asset a, const VbtFrameIndex& ix, int t0, int nt, float* d_out, hipStream_t s)
+{
+    const AssetDev ad = a->dev();
+    if (ix.wide)
+        k_cache_indexed<true><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(ad, ix.off, ix.rec, ix.bins, t0, nt, d_out,
+                                                                                  a->controlsPerFrame, a->d_err);
+    else
+        k_cache_indexed<false><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(ad, ix.off, ix.rec, ix.bins, t0, nt, d_out,
+                                                                                   a->controlsPerFrame, a->d_err);
+}
+
+// Records that a build on stream s reads the asset's own index.
+int note_index_use(vbt_asset a, hipStream_t s)
+{
+    for (auto& u : a->indexUse)
+        if (u.first == s) {
+            HIPCHK(hipEventRecord(u.second, s));
+            return VBT_OK;
+        }
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    a->indexUse.push_back({s, ev});
+    HIPCHK(hipEventRecord(ev, s));
+    return VBT_OK;
+}
+
+} // namespace
+
+int vbt_internal_frame_cache_indexed(vbt_asset a, const VbtFrameIndex& ix, int t0, int nt, float* d_out, void* stream)
+{
+    if (!a || !d_out || nt <= 0 || nt > CACHE_EXACT_MAX_FRAMES || !ix.off)
+        return vbt_fail(VBT_ERR_INVALID, "invalid indexed frame cache request");
+    if (t0 < 0 || (uint64_t)t0 + (uint64_t)nt > a->h.frames) return vbt_fail(VBT_ERR_INVALID, "frame range outside the asset");
+    if (a->A == 0) return a->bad ? report_bad(a, "frame cache") : VBT_OK;
+    launch_indexed(a, ix, t0, nt, d_out, S(stream));
+    HIPCHK(hipGetLastError());
+    if (a->bad) return report_bad(a, "frame cache");
+    return VBT_OK;
+}
+
+void vbt_internal_frame_index_free(VbtFrameIndex& ix)
+{
+    if (ix.off) (void)hipFree(ix.off);
+    if (ix.rec) (void)hipFree(ix.rec);
+    ix = VbtFrameIndex{};
+}
+
+int vbt_internal_frame_index_build(vbt_asset a, VbtFrameIndex& ix, void* stream)
+{
+    if (!a) return vbt_fail(VBT_ERR_INVALID, "frame index: null asset");
+    if (a->h.profileType == 2) return vbt_fail(VBT_ERR_INVALID, "frame index: render assets only");
+    if (ix.off || ix.rec) return vbt_fail(VBT_ERR_INVALID, "frame index: already built");
+    const uint32_t B = index_bins(a->h.frames);
+    const uint64_t C = a->controlsPerFrame, N = (uint64_t)B * (C + 1u);
+    if (a->A == 0 || C == 0) return VBT_OK;
+    if (N > 0x7FFFFFFFull) return vbt_fail(VBT_ERR_INVALID, "frame index: too many controls");
+    HIPCHK(hipSetDevice(a->device));
+    hipStream_t s = S(stream);
+    const AssetDev ad = a->dev();
+    uint32_t* cnt = nullptr;
+    void* tmp = nullptr;
+    size_t tmpBytes = 0;
+    uint32_t last = 0;
+    auto fail = [&](int code, const char* msg) {
+        (void)hipStreamSynchronize(s);
+        if (cnt) (void)hipFree(cnt);
+        if (tmp) (void)hipFree(tmp);
+        vbt_internal_frame_index_free(ix);
+        return vbt_fail(code, msg);
+    };
+    if (hipMalloc((void**)&cnt, 4ull * N) != hipSuccess || hipMalloc((void**)&ix.off, 4ull * N) != hipSuccess)
+        return fail(VBT_ERR_NOMEM, "frame index: out of device memory");
+    // d_err[1]: the fp32 flag, d_err[2..3]: the 64-bit record total
+    if (hipMemsetAsync(a->d_err + 1, 0, 12, s) != hipSuccess) return fail(VBT_ERR_DEVICE, "frame index: clear failed");
+    k_index_count<<<blocks_for(C + 1u, 256), 256, 0, s>>>(ad, cnt, C, B, a->d_err + 1);
+    // the record total must fit the 32-bit offsets (each prefix is at most it)
+    unsigned long long sum = 0;
+    uint32_t wide = 0;
+    k_sum64<<<1024, 256, 0, s>>>(cnt, N, (unsigned long long*)(a->d_err + 2));
+    if (hipMemcpyAsync(&sum, a->d_err + 2, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&wide, a->d_err + 1, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(VBT_ERR_DEVICE, "frame index: count failed");
+    if (sum > 0xFFFFFFFFull) return fail(VBT_ERR_INVALID, "frame index: more than 2^32 records");
+    if (sum == 0) { // nothing indexable: builds keep the bracket search
+        (void)hipFree(cnt);
+        vbt_internal_frame_index_free(ix);
+        return VBT_OK;
+    }
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, cnt, ix.off, (int)N, s) != hipSuccess ||
+        hipMalloc(&tmp, tmpBytes ? tmpBytes : 4) != hipSuccess ||
+        hipcub::DeviceScan::ExclusiveSum(tmp, tmpBytes, cnt, ix.off, (int)N, s) != hipSuccess ||
+        hipMemcpyAsync(&last, ix.off + (N - 1u), 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(VBT_ERR_DEVICE, "frame index: scan failed");
+    // the slot count after each bin's last slot is 0, so `last` is the total
+    const uint64_t total = last;
+    const uint64_t recBytes = wide ? 8u : 4u;
+    if (hipMalloc(&ix.rec, total * recBytes) != hipSuccess) return fail(VBT_ERR_NOMEM, "frame index: out of device memory");
+    if (wide) k_index_scatter<true><<<blocks_for(C, 256), 256, 0, s>>>(ad, ix.off, ix.rec, C, B);
+    else k_index_scatter<false><<<blocks_for(C, 256), 256, 0, s>>>(ad, ix.off, ix.rec, C, B);
+    const hipError_t e = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(cnt);
+    (void)hipFree(tmp);
+    cnt = nullptr;
+    tmp = nullptr;
+    if (e != hipSuccess || e2 != hipSuccess) return fail(VBT_ERR_DEVICE, hipGetErrorString(e != hipSuccess ? e : e2));
+    ix.records = total;
+    ix.bins = B;
+    ix.wide = wide != 0;
+    ix.bytes = 4ull * N + recBytes * total;
+    return VBT_OK;
+}
 
 // accessors for the serving layer (server.hip)
 vbtk::AssetDev vbt_internal_asset_dev(vbt_asset a) { return a->dev(); }
@@ -1146,13 +1270,12 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     if (a->h.profileType == 2) {
         k_sci_leaf<false><<<dim3(a->A, (uint32_t)nt), 512, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
     } else if (layout == VBT_CACHE_PACKED) {
-        if (nt <= CACHE_EXACT_MAX_FRAMES && a->d_binOff && a->binWide)
-            k_cache_indexed<true><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
-                ad, a->d_binOff, a->d_binRec, t0, nt, d_out, a->controlsPerFrame, a->d_err);
-        else if (nt <= CACHE_EXACT_MAX_FRAMES && a->d_binOff)
-            k_cache_indexed<false><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
-                ad, a->d_binOff, a->d_binRec, t0, nt, d_out, a->controlsPerFrame, a->d_err);
-        else if (nt <= CACHE_EXACT_MAX_FRAMES)
+        if (nt <= CACHE_EXACT_MAX_FRAMES && a->index.off) {
+            launch_indexed(a, a->index, t0, nt, d_out, s);
+            HIPCHK(hipGetLastError());
+            int rc = note_index_use(a, s);
+            if (rc) return rc;
+        } else if (nt <= CACHE_EXACT_MAX_FRAMES)
             k_cache_exact<VBT_CACHE_PACKED><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
                 ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
         else
@@ -1264,17 +1387,19 @@ int vbt_cache_tile_shape(int tile, uint32_t* block, uint32_t* window)
 int vbt_asset_drop_frame_index(vbt_asset a)
 {
     if (!a) return vbt_fail(VBT_ERR_INVALID, "vbt_asset_drop_frame_index: null asset");
-    if (a->d_binOff || a->d_binRec) {
-        HIPCHK(hipSetDevice(a->device));
-        HIPCHK(hipDeviceSynchronize());
+    int rc = VBT_OK;
+    if (a->index.off || a->index.rec) {
+        // builds that read the index may still run: wait for each stream's last one
+        for (auto& u : a->indexUse) {
+            const hipError_t e = hipEventSynchronize(u.second);
+            if (e != hipSuccess && rc == VBT_OK) rc = vbt_fail(VBT_ERR_DEVICE, "frame index: wait for its readers");
+        }
+        a->deviceBytes -= a->index.bytes;
+        vbt_internal_frame_index_free(a->index);
     }
-    if (a->d_binOff) (void)hipFree(a->d_binOff);
-    if (a->d_binRec) (void)hipFree(a->d_binRec);
-    a->d_binOff = nullptr;
-    a->d_binRec = nullptr;
-    a->binRecords = 0;
-    a->binWide = false;
-    return VBT_OK;
+    for (auto& u : a->indexUse) (void)hipEventDestroy(u.second);
+    a->indexUse.clear();
+    return rc;
 }
 
 int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream)
@@ -1283,82 +1408,10 @@ int vbt_asset_build_frame_index(vbt_asset a, uint64_t* bytes, void* stream)
     if (a->h.profileType == 2) return vbt_fail(VBT_ERR_INVALID, "frame index: render assets only");
     int rc = vbt_asset_drop_frame_index(a);
     if (rc) return rc;
-    const uint64_t C = a->controlsPerFrame, N = (uint64_t)INDEX_BINS * (C + 1u);
     if (bytes) *bytes = 0;
-    if (a->A == 0 || C == 0) return VBT_OK;
-    if (N > 0x7FFFFFFFull) return vbt_fail(VBT_ERR_INVALID, "frame index: too many controls");
-    HIPCHK(hipSetDevice(a->device));
-    hipStream_t s = S(stream);
-    const AssetDev ad = a->dev();
-    uint32_t* cnt = nullptr;
-    void* tmp = nullptr;
-    size_t tmpBytes = 0;
-    uint32_t last = 0;
-    auto cleanup = [&]() {
-        if (cnt) (void)hipFree(cnt);
-        if (tmp) (void)hipFree(tmp);
-    };
-    if (hipMalloc((void**)&cnt, 4ull * N) != hipSuccess || hipMalloc((void**)&a->d_binOff, 4ull * N) != hipSuccess) {
-        cleanup();
-        vbt_asset_drop_frame_index(a);
-        return vbt_fail(VBT_ERR_NOMEM, "frame index: out of device memory");
-    }
-    // d_err[1]: the fp32 flag, d_err[2..3]: the 64-bit record total
-    if (hipMemsetAsync(a->d_err + 1, 0, 12, s) != hipSuccess) {
-        cleanup();
-        vbt_asset_drop_frame_index(a);
-        return vbt_fail(VBT_ERR_DEVICE, "frame index: clear failed");
-    }
-    k_index_count<<<blocks_for(C + 1u, 256), 256, 0, s>>>(ad, cnt, C, a->d_err + 1);
-    // the record total must fit the 32-bit offsets (each prefix is at most it)
-    unsigned long long sum = 0;
-    uint32_t wide = 0;
-    k_sum64<<<1024, 256, 0, s>>>(cnt, N, (unsigned long long*)(a->d_err + 2));
-    if (hipMemcpyAsync(&sum, a->d_err + 2, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(&wide, a->d_err + 1, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-        cleanup();
-        vbt_asset_drop_frame_index(a);
-        return vbt_fail(VBT_ERR_DEVICE, "frame index: count failed");
-    }
-    if (sum > 0xFFFFFFFFull) {
-        cleanup();
-        vbt_asset_drop_frame_index(a);
-        return vbt_fail(VBT_ERR_INVALID, "frame index: more than 2^32 records");
-    }
-    if (sum == 0) {
-        cleanup();
-        return vbt_asset_drop_frame_index(a); // nothing indexable: builds keep the bracket search
-    }
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, cnt, a->d_binOff, (int)N, s) != hipSuccess ||
-        hipMalloc(&tmp, tmpBytes ? tmpBytes : 4) != hipSuccess ||
-        hipcub::DeviceScan::ExclusiveSum(tmp, tmpBytes, cnt, a->d_binOff, (int)N, s) != hipSuccess ||
-        hipMemcpyAsync(&last, a->d_binOff + (N - 1u), 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-        cleanup();
-        vbt_asset_drop_frame_index(a);
-        return vbt_fail(VBT_ERR_DEVICE, "frame index: scan failed");
-    }
-    // the slot count after each bin's last slot is 0, so `last` is the total
-    const uint64_t total = last;
-    const uint64_t recBytes = wide ? 8u : 4u;
-    if (hipMalloc(&a->d_binRec, total * recBytes) != hipSuccess) {
-        cleanup();
-        vbt_asset_drop_frame_index(a);
-        return vbt_fail(VBT_ERR_NOMEM, "frame index: out of device memory");
-    }
-    if (wide) k_index_scatter<true><<<blocks_for(C, 256), 256, 0, s>>>(ad, a->d_binOff, a->d_binRec, C);
-    else k_index_scatter<false><<<blocks_for(C, 256), 256, 0, s>>>(ad, a->d_binOff, a->d_binRec, C);
-    const hipError_t e = hipGetLastError();
-    (void)hipStreamSynchronize(s);
-    cleanup();
-    if (e != hipSuccess) {
-        vbt_asset_drop_frame_index(a);
-        return vbt_fail(VBT_ERR_DEVICE, hipGetErrorString(e));
-    }
-    a->binRecords = total;
-    a->binWide = wide != 0;
-    if (bytes) *bytes = 4ull * N + recBytes * total;
+    if ((rc = vbt_internal_frame_index_build(a, a->index, stream))) return rc;
+    a->deviceBytes += a->index.bytes; // counted in the asset's device memory (and so in server budgets)
+    if (bytes) *bytes = a->index.bytes;
     return VBT_OK;
 }
 

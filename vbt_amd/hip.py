@@ -79,7 +79,7 @@ def march_params(**kw) -> MarchParams:
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbmBudgetBytes", c_u64), ("arenaBytes", c_u64), ("stagingBytes", c_u64),
                 ("maxInflight", c_u32), ("blocksPerCU", c_u32), ("queryItem", c_u32), ("cacheItem", c_u32),
-                ("idleUs", c_u32), ("leaseMs", c_u32), ("frameBuildsInLoop", c_u32)]
+                ("idleUs", c_u32), ("leaseMs", c_u32), ("frameBuildsInLoop", c_u32), ("frameIndex", c_u32)]
 
 
 class RequestResult(ctypes.Structure):
