@@ -256,6 +256,70 @@ def query_bench(asset, hip, n_batch: int, reps: int) -> dict:
     return res
 
 
+def query_credit_bytes(data: bytes, q: np.ndarray) -> float:
+    """Credited bytes of a batch of point queries (render assets): 20 in, 4 out,
+    12 of offset, header word and prefix, then per trilinear corner control
+    (8 coarse, plus 8 fine for Fine6 leaves) a descriptor, a bin hint and two
+    keyframe records; a Mode2 leaf adds its 80-byte occupancy section and one
+    shell control.  Empty leaves cost the first 36 bytes only."""
+    hdr = np.frombuffer(data, np.uint32, 11, 8)
+    W, H, D, F, leaf_size, L, scales = (int(hdr[i]) for i in (1, 2, 3, 4, 5, 8, 10))
+    base = 52 + 4 * scales
+    offs = np.frombuffer(data, np.uint32, L + 1, base)
+    words = np.frombuffer(data, np.uint32, offset=base + 4 * (L + 1))
+    lcx, lcy, lcz = -(-W // leaf_size), -(-H // leaf_size), -(-D // leaf_size)
+    bx = np.minimum(q[:, 0] // leaf_size, lcx - 1)
+    by = np.minimum(q[:, 1] // leaf_size, lcy - 1)
+    bz = np.minimum(q[:, 2] // leaf_size, lcz - 1)
+    head = words[offs[(bz * lcy + by) * lcx + bx]]
+    mode, cc, fc = head & 3, (head >> 2) & 3, (head >> 4) & 3
+    kb = lambda c: np.where(c == 1, 8, 3)  # noqa: E731
+    per = 36.0 + np.where(mode != 0, 8 * (4 + 2 + 2 * kb(cc)), 0)
+    per = per + np.where(mode == 3, 8 * (4 + 2 + 2 * kb(fc)), 0) + np.where(mode == 2, 80 + 4 + 2 + 2 * kb(fc), 0)
+    return float(per.sum())
+
+
+def pattern_bench(hip, asset, data: bytes, reps: int = 50) -> dict:
+    """Point queries in the reference's request shapes (render/src/query_patterns.cpp:
+    random, same-t, same-xyz, coherent 8x8x8 tiles), as generated and in
+    leaf-major order, 65,536 per batch: the batched kernel (8 lanes per
+    query) against the plain one-thread-per-query kernel, HIP events around
+    each launch; effective GB/s over the credited bytes."""
+    s = asset.stream
+    out = {}
+    n = 65536
+    dout = hip.DeviceArray(n, np.float32, s)
+    for pattern in ("random", "same-t", "same-xyz", "coherent"):
+        for order in ("generated", "leaf-sorted"):
+            q = hip.generate_queries(asset.header, pattern, n, 0x5EED0001, sort_by_leaf=order == "leaf-sorted")
+            credit = query_credit_bytes(data, q)
+            dq = hip.DeviceArray.from_host(q, s)
+            rec = {}
+            for kind, plain in (("batched", False), ("plain", True)):
+                for _ in range(3):
+                    asset.query_into(dq, n, dout, plain=plain)
+                s.sync()
+                lat = []
+                e0, e1 = hip.Event(), hip.Event()
+                for _ in range(reps):
+                    e0.record(s)
+                    asset.query_into(dq, n, dout, plain=plain)
+                    e1.record(s)
+                    lat.append(e0.elapsed_ms(e1) * 1e3)
+                med = float(np.median(lat))
+                rec[kind] = {"qps": round(n / (med * 1e-6), 1), "p50_us": round(float(np.percentile(lat, 50)), 2),
+                             "p99_us": round(float(np.percentile(lat, 99)), 2),
+                             "effective_gbs": round(credit / (med * 1e-6) / 1e9, 1)}
+            rec["credited_bytes_per_query"] = round(credit / n, 1)
+            rec["batched_over_plain"] = round(rec["batched"]["qps"] / rec["plain"]["qps"], 2)
+            out[f"{pattern}/{order}"] = rec
+            dq.free()
+    dout.free()
+    print("[bench] query patterns: " + ", ".join(f"{k} {v['batched']['qps'] / 1e9:.2f} G/s ({v['batched_over_plain']}x)"
+                                                 for k, v in out.items()), file=sys.stderr, flush=True)
+    return out
+
+
 def timed(hip, stream, fn, steps: int, warmup: int = 1) -> float:
     """Mean ms per call of fn() on `stream` (HIP events around `steps` calls)."""
     for _ in range(warmup):
@@ -268,6 +332,35 @@ def timed(hip, stream, fn, steps: int, warmup: int = 1) -> float:
     e1.record(stream)
     stream.sync()
     return e0.elapsed_ms(e1) / steps
+
+
+def frames_bench(hip, asset, workload: str, steps: int) -> dict:
+    """Every frame of one asset into its packed cache in one launch, HIP
+    events on the launch stream: the liquid level set (Mode2 shells, the
+    sweep kernel with its autotuned tile) and the scientific field (the
+    temporal-DCT leaf kernel, one workgroup per leaf and frame)."""
+    info = asset.info
+    F = asset.dims[3]
+    s = asset.stream
+    tile = None
+    if info.header.profileType != 2:
+        tile = asset.autotune(F, reps=1)[0]
+    out = hip.DeviceArray((F, info.controlsPerFrame), np.float32, s)
+    ms = timed(hip, s, lambda: asset.frame_cache_into(out, 0, F), max(3, steps // 2), 1)
+    out.free()
+    byt = algorithmic_bytes(info, F)
+    shapes = hip.Asset.tile_shapes()
+    kname = (f"k_cache_render<0, {shapes[tile][0]}, {shapes[tile][1]}>" if tile is not None else "k_sci_leaf<false>")
+    achieved = byt["total"] / (ms * 1e-3) / 1e9
+    res = {"workload": f"{workload} {'x'.join(map(str, asset.dims[:3]))} x {F} frames, all frames per launch",
+           "kernel": kname, "ms": round(ms, 4), "frames": F, "slots_per_frame": info.controlsPerFrame,
+           "bytes_per_launch": byt["total"], "written_per_launch": byt["written"],
+           "decoded_gbs": round(byt["written"] / (ms * 1e-3) / 1e9, 1),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload, kname)}}
+    print(f"[bench] all frames of {workload}: {ms:.4f} ms, {res['decoded_gbs']} GB/s decoded, "
+          f"{100 * achieved / HBM_PEAK_GBS:.1f} % of HBM peak ({kname})", file=sys.stderr, flush=True)
+    return res
 
 
 def batch_bench(hip, assets: dict, names, steps: int, tune_reps: int = 2) -> dict:
@@ -555,6 +648,9 @@ def main():
     ap.add_argument("--no-ceilings", action="store_true")
     ap.add_argument("--no-march", action="store_true")
     ap.add_argument("--no-cycles", action="store_true")
+    ap.add_argument("--frames-assets", default="liquid320,sci192",
+                    help="assets whose every frame is built in one launch (the Mode2 and scientific kernels)")
+    ap.add_argument("--no-frames", action="store_true")
     ap.add_argument("--batch-tune-reps", type=int, default=2)
     ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
                     help="assets of the batched frame-cache build (all frames of each, one graph replay)")
@@ -660,9 +756,11 @@ def main():
         asset.drop_frame_index()
 
     qres = small = None
+    patterns = None
     if not args.no_queries:
         qres = query_bench(asset, hip, 65536, 50)
         small = query_bench(asset, hip, 256, 200)
+        patterns = pattern_bench(hip, asset, data)
 
     kname = f"k_cache_render<0, {shapes[best][0]}, {shapes[best][1]}>"
     line = {
@@ -690,13 +788,14 @@ def main():
                      "bytes_per_launch": byt["total"],
                      "kernel": kname,
                      "device_ceilings_gbs": ceil},
-        "queries": {"random_65536": qres, "random_256": small},
+        "queries": {"random_65536": qres, "random_256": small, "patterns_65536": patterns},
         "single_frame": single,
     }
     extra = {args.workload: asset}
     raws = {args.workload: data}
     want = ([] if args.no_batch else [x for x in args.batch_assets.split(",") if x]) + \
         ([] if args.no_serve else [x for x in args.serve_assets.split(",") if x]) + \
+        ([] if args.no_frames else [x for x in args.frames_assets.split(",") if x]) + \
         ([] if args.no_march and args.no_cycles else ["liquid320"])
     for n in want:
         if n not in extra:
@@ -717,6 +816,9 @@ def main():
         c1 = line["cfg1_smoke64"]
         print(f"[bench] cfg1 smoke64: GPU {c1['gpu_ms']} ms, CPU {c1.get('cpu_port_1_thread_ms')} ms (1 thread)",
               file=sys.stderr, flush=True)
+    if not args.no_frames:
+        line["all_frames"] = {n: frames_bench(hip, extra[n], n, args.steps)
+                              for n in [x for x in args.frames_assets.split(",") if x]}
     if not args.no_batch:
         line["batch_single"] = batch_bench(hip, extra, [args.workload], args.steps, args.batch_tune_reps)
         names = [x for x in args.batch_assets.split(",") if x]

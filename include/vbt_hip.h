@@ -203,6 +203,25 @@ typedef struct {
 /* decodeScientificHeaderV2, render/src/vbt_file.h:62 */
 void vbt_decode_scientific_header(uint32_t packedHeader, vbt_scientific_header* out);
 
+/* Query request shapes, render/src/query_patterns.h:19-33 (host).  Queries
+ * are (x, y, z, t) rows of four u32, the layout vbt_query_points reads. */
+enum {
+    VBT_QUERY_RANDOM = 0,   /* QueryPattern::RandomFull: uniform x, y, z, t */
+    VBT_QUERY_SAME_T = 1,   /* SameT: uniform x, y, z at one frame */
+    VBT_QUERY_SAME_XYZ = 2, /* SameXYZ: one voxel at uniform frames */
+    VBT_QUERY_COHERENT = 3, /* CoherentTiles: 8x8x8 tiles of one random leaf-aligned corner per query, one frame */
+};
+/* parseQueryPattern, query_patterns.h:28 ("random", "same-t", "same-xyz",
+ * "coherent"); VBT_ERR_INVALID for other text. */
+int vbt_parse_query_pattern(const char* text, int* pattern);
+/* generateQueries, query_patterns.h:29-32: the same queries as the reference
+ * for the same header, pattern, count and seed (std::mt19937 seeded with
+ * `seed`, std::uniform_int_distribution per axis).  out: 4 * count u32. */
+int vbt_generate_queries(const vbt_file_header* h, int pattern, uint32_t count, uint32_t seed, uint32_t* out);
+/* sortQueriesByLeaf, query_patterns.h:34: stable sort by (leaf, t, z, y, x),
+ * the leaf from leafIndexForVoxel. */
+int vbt_sort_queries_by_leaf(const vbt_file_header* h, uint32_t* queries, uint32_t n);
+
 typedef struct vbt_asset_s* vbt_asset;
 
 /* Validate (as loadVbtFile), upload the offset table, payload pool and AC

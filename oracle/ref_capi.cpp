@@ -6,6 +6,7 @@
 // controls the way the reference's CPU frame decode does), catches their
 // exceptions and returns the message.  It lets the tests check the oracle restatement against the
 // real reference implementation on the same bytes.
+#include "query_patterns.h"
 #include "render_temporal_decode.h"
 #include "render_temporal_payload.h"
 #include "scientific_decode.h"
@@ -201,6 +202,36 @@ int vbtref_cache_frame(void* h, int t, const uint64_t* ctrlOffset, float* out)
                     if ((occ.shellMask[vox >> 6] >> (vox & 63u)) & 1u)
                         *o++ = vbt::decodeRenderTemporalLeafShellVoxelValue(leaf, v, true, frames, vox, t);
             }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// render/src/query_patterns.cpp:29-101: generateQueries for a header of
+// these dims, then optionally sortQueriesByLeaf (after fillLeafIndices);
+// out: (x, y, z, t) rows
+int vbtref_generate_queries(uint32_t width, uint32_t height, uint32_t depth, uint32_t frames, int pattern,
+                            uint32_t count, uint32_t seed, int sortByLeaf, uint32_t* out)
+{
+    try {
+        vbt::render::VbtFileHeader h{};
+        h.width = width;
+        h.height = height;
+        h.depth = depth;
+        h.frames = frames;
+        h.leafSize = 8;
+        auto q = vbt::render::generateQueries(h, static_cast<vbt::render::QueryPattern>(pattern), count, seed);
+        if (sortByLeaf) {
+            vbt::render::fillLeafIndices(h, q);
+            vbt::render::sortQueriesByLeaf(q);
+        }
+        for (size_t i = 0; i < q.size(); ++i) {
+            out[4 * i] = q[i].x;
+            out[4 * i + 1] = q[i].y;
+            out[4 * i + 2] = q[i].z;
+            out[4 * i + 3] = q[i].t;
         }
         return 0;
     } catch (const std::exception& e) {

@@ -10,6 +10,7 @@
 //          s <leaf> <voxel> <frame>               decodeRenderTemporalLeafShellVoxelValue
 //          q <x> <y> <z> <t>                      decodeScientificValueAtCpu
 //          f <t>                                  reconstructScientificFrameCpu (checksum)
+#include "../../include/vbt/query_patterns.h"
 #include "../../include/vbt/render_temporal_decode.h"
 #include "../../include/vbt/scientific_decode.h"
 #include "../../include/vbt/vbt_file.h"
@@ -52,6 +53,18 @@ static int host(const VbtFile& f)
     const uint32_t vox[6][3] = {{0, 0, 0}, {h.width - 1, 0, 0}, {0, h.height - 1, h.depth - 1},
                                 {h.width, h.height, h.depth}, {7, 8, 9}, {h.width / 2, h.height / 3, h.depth / 4}};
     for (auto& v : vox) std::printf("{\"leaf_of\": [%u, %u, %u, %u]}\n", v[0], v[1], v[2], leafIndexForVoxel(h, v[0], v[1], v[2]));
+    // request shapes as the reference's query tools make them: generate, then
+    // leaf indices and the leaf-major order
+    for (const char* name : {"random", "same-t", "same-xyz", "coherent"}) {
+        QueryPattern p;
+        if (!parseQueryPattern(name, p)) std::printf("{\"error\": \"pattern %s\"}\n", name);
+        auto q = generateQueries(h, p, 1000, 77u);
+        fillLeafIndices(h, q);
+        sortQueriesByLeaf(q);
+        uint64_t qs = 0;
+        for (const auto& r : q) qs = ((((qs * 1000003u + r.x) * 1000003u + r.y) * 1000003u + r.z) * 1000003u + r.t) * 1000003u + r.leafIndex;
+        std::printf("{\"queries\": [\"%s\", %" PRIu64 "]}\n", name, qs);
+    }
     for (uint32_t l = 0; l < h.leafCount; ++l) {
         const uint32_t word = readPackedHeaderWord(f, l);
         if (h.profileType == 2) {

@@ -153,6 +153,8 @@ def ref():
         L.vbtref_sci_frame.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
         L.vbtref_cache_frame.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.vbtref_sci_values.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+        L.vbtref_generate_queries.argtypes = [ctypes.c_uint32] * 4 + [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                                                      ctypes.c_int, ctypes.c_void_p]
         _R = L
     return _R
 

@@ -78,6 +78,28 @@ def test_shim_container_and_leaf_format(files):
         assert lines[-1]["prefix_error"] == "render temporal prefix keyframe count exceeds uint16"
 
 
+def test_shim_query_patterns(files):
+    """generateQueries + fillLeafIndices + sortQueriesByLeaf through the C++
+    drop-in equal the C-ABI's generator and leaf-major sort (which the CPU
+    unit tests pin to the reference's query_patterns.cpp)."""
+    from vbt_amd import hip
+
+    O = oracle_py.oracle()
+    for name, (path, o) in files.items():
+        got = {l["queries"][0]: l["queries"][1] for l in run("host", path) if "queries" in l}
+        hdr = hip.FileHeader()
+        for f, _ in hdr._fields_:
+            setattr(hdr, f, getattr(o.header, f))
+        for pattern in ("random", "same-t", "same-xyz", "coherent"):
+            q = hip.generate_queries(hdr, pattern, 1000, 77, sort_by_leaf=True)
+            s = 0
+            for x, y, z, t in q.tolist():
+                leaf = O.vbto_leaf_index_for_voxel(ctypes.byref(o.vf.header), x, y, z)
+                for v in (x, y, z, t, leaf):
+                    s = (s * 1000003 + v) & MASK
+            assert got[pattern] == s, (name, pattern)
+
+
 def test_shim_load_error_text(tmp_path):
     data = bytearray(golden_io.vbtp("smoke64"))
     data[8:12] = (5).to_bytes(4, "little")  # version 5

@@ -523,3 +523,33 @@ def test_scientific_queries_equal_the_reference_cpu_query_loop(name):
                   rng.integers(0, F, 5000)], axis=1).astype(np.uint32)
     assert (rf.sci_values(q).view(np.uint32) == of.query(q).view(np.uint32)).all()
     rf.close()
+
+
+@need_ref
+@pytest.mark.parametrize("pattern", ["random", "same-t", "same-xyz", "coherent"])
+@pytest.mark.parametrize("dims", [(64, 64, 64, 32), (37, 21, 9, 5), (256, 256, 256, 120)])
+def test_query_patterns_equal_the_reference(pattern, dims):
+    """vbt_generate_queries / vbt_sort_queries_by_leaf against the reference's
+    generateQueries and sortQueriesByLeaf (render/src/query_patterns.cpp,
+    compiled into oracle/_ref) for several seeds, unsorted and leaf-sorted."""
+    from vbt_amd import hip
+
+    W, H, D, F = dims
+    hdr = hip.FileHeader()
+    hdr.width, hdr.height, hdr.depth, hdr.frames, hdr.leafSize = W, H, D, F, 8
+    p = hip.QUERY_PATTERNS[pattern]
+    for seed in (1, 0x5EED0001, 12345):
+        for srt in (False, True):
+            got = hip.generate_queries(hdr, pattern, 3000, seed, sort_by_leaf=srt)
+            exp = np.zeros((3000, 4), np.uint32)
+            assert oracle_py.ref().vbtref_generate_queries(W, H, D, F, p, 3000, seed, int(srt), exp.ctypes.data) == 0
+            assert (got == exp).all(), (pattern, dims, seed, srt)
+    # shapes: same-t fixes t, same-xyz fixes the voxel, coherent tiles stay in one frame
+    q = hip.generate_queries(hdr, pattern, 512, 7)
+    if pattern in ("same-t", "coherent"):
+        assert len(set(q[:, 3])) == 1
+    if pattern == "same-xyz":
+        assert len({tuple(r) for r in q[:, :3]}) == 1
+    assert (q[:, 0] < W).all() and (q[:, 1] < H).all() and (q[:, 2] < D).all() and (q[:, 3] < F).all()
+    with pytest.raises(hip.VbtError):
+        hip.generate_queries(hdr, "diagonal", 4, 1)

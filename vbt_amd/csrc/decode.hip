@@ -402,9 +402,9 @@ __global__ void __launch_bounds__(256) k_cache_exact(AssetDev a, int t0, int nt,
 // the reference bracket search (cache_slot_exact).
 // Bins of the frame range the index keeps runs for: a one-frame build reads
 // its bin's run of every slot (about keys / bins + 2 records), the index holds
-// about keys + 2 x bins records per slot.  32 bins: liquid320 one-frame builds
-// 99 -> 58 us against 8 bins, for an index of 4.1 x the payload (8 bins:
-// 1.8 x).  At most one bin per frame.
+// about keys + 2 x bins records per slot.  32 bins against 8: liquid320
+// one-frame builds 99 -> 63 us, smoke256 15 -> 13 us, for an index of 3.7 x
+// the payload instead of 1.8 x.  At most one bin per frame.
 constexpr uint32_t INDEX_BINS_MAX = 32u;
 __host__ __device__ inline uint32_t index_bins(uint32_t F) { return F < INDEX_BINS_MAX ? (F ? F : 1u) : INDEX_BINS_MAX; }
 

@@ -9,8 +9,10 @@
 #include "vbt_format.h"
 #include "vbt_internal.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -442,6 +444,96 @@ const char* vbt_status_message(int32_t s)
     case VBT_ERR_CONTROL_INDEX: return "render temporal control index out of range";
     default: return "unknown status";
     }
+}
+
+} // extern "C"
+
+// ------------------------------------------------------- query patterns --
+// render/src/query_patterns.cpp:8-101.  The generator draws from the same
+// standard engine and distributions in the same order as the reference, so
+// a seed gives the reference's queries.
+extern "C" {
+
+int vbt_parse_query_pattern(const char* text, int* pattern)
+{
+    if (!text || !pattern) return vbt_fail(VBT_ERR_INVALID, "null query pattern text or output");
+    static const char* names[] = {"random", "same-t", "same-xyz", "coherent"};
+    for (int i = 0; i < 4; ++i)
+        if (!std::strcmp(text, names[i])) {
+            *pattern = i;
+            return VBT_OK;
+        }
+    return vbt_fail(VBT_ERR_INVALID, std::string("unknown query pattern: ") + text);
+}
+
+int vbt_generate_queries(const vbt_file_header* h, int pattern, uint32_t count, uint32_t seed, uint32_t* out)
+{
+    if (!h || (!out && count)) return vbt_fail(VBT_ERR_INVALID, "null header or output");
+    if (pattern < VBT_QUERY_RANDOM || pattern > VBT_QUERY_COHERENT)
+        return vbt_fail(VBT_ERR_INVALID, "unknown query pattern");
+    std::mt19937 rng(seed);
+    auto axis = [](uint32_t n) { return std::uniform_int_distribution<uint32_t>(0u, std::max(1u, n) - 1u); };
+    auto dx = axis(h->width), dy = axis(h->height), dz = axis(h->depth), dt = axis(h->frames);
+    const uint32_t fx = dx(rng), fy = dy(rng), fz = dz(rng), ft = dt(rng);
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t* q = out + 4ull * i;
+        switch (pattern) {
+        case VBT_QUERY_RANDOM:
+            q[0] = dx(rng);
+            q[1] = dy(rng);
+            q[2] = dz(rng);
+            q[3] = dt(rng);
+            break;
+        case VBT_QUERY_SAME_T:
+            q[0] = dx(rng);
+            q[1] = dy(rng);
+            q[2] = dz(rng);
+            q[3] = ft;
+            break;
+        case VBT_QUERY_SAME_XYZ:
+            q[0] = fx;
+            q[1] = fy;
+            q[2] = fz;
+            q[3] = dt(rng);
+            break;
+        default: {
+            const uint32_t bx = dx(rng) / 8u * 8u, by = dy(rng) / 8u * 8u, bz = dz(rng) / 8u * 8u;
+            q[0] = std::min(h->width - 1u, bx + i % 8u);
+            q[1] = std::min(h->height - 1u, by + (i / 8u) % 8u);
+            q[2] = std::min(h->depth - 1u, bz + (i / 64u) % 8u);
+            q[3] = ft;
+        }
+        }
+    }
+    return VBT_OK;
+}
+
+int vbt_sort_queries_by_leaf(const vbt_file_header* h, uint32_t* queries, uint32_t n)
+{
+    if (!h || (!queries && n)) return vbt_fail(VBT_ERR_INVALID, "null header or queries");
+    struct Q {
+        uint32_t leaf, x, y, z, t;
+    };
+    std::vector<Q> v(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t* q = queries + 4ull * i;
+        v[i] = Q{vbt_leaf_index_for_voxel(h, q[0], q[1], q[2]), q[0], q[1], q[2], q[3]};
+    }
+    std::stable_sort(v.begin(), v.end(), [](const Q& a, const Q& b) {
+        if (a.leaf != b.leaf) return a.leaf < b.leaf;
+        if (a.t != b.t) return a.t < b.t;
+        if (a.z != b.z) return a.z < b.z;
+        if (a.y != b.y) return a.y < b.y;
+        return a.x < b.x;
+    });
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t* q = queries + 4ull * i;
+        q[0] = v[i].x;
+        q[1] = v[i].y;
+        q[2] = v[i].z;
+        q[3] = v[i].t;
+    }
+    return VBT_OK;
 }
 
 } // extern "C"

@@ -7,10 +7,12 @@
 // functions with the same types, so either set compiles against it.
 // (validateRenderTemporalLeafPrefix is the exception: it takes the encoder's
 // option struct, so it stays an inline function of include/vbt/.)
+#include "../../include/vbt/query_patterns.h"
 #include "../../include/vbt/render_temporal_decode.h"
 #include "../../include/vbt/scientific_decode.h"
 #include "../../include/vbt/vbt_file.h"
 
+#include <algorithm>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -449,6 +451,52 @@ std::vector<float> reconstructScientificFrameCpu(const VbtFile& file, uint32_t f
     vbt::hip::DeviceBuffer<float> out(static_cast<size_t>(h.width) * h.height * h.depth);
     asset.denseFrame(static_cast<int>(frameIndex), out.data());
     return out.download();
+}
+
+} // namespace vbt::render
+
+// ---- query_patterns.h
+namespace vbt::render {
+
+bool parseQueryPattern(const std::string& text, QueryPattern& outPattern)
+{
+    int p = 0;
+    if (vbt_parse_query_pattern(text.c_str(), &p) != VBT_OK) return false;
+    outPattern = static_cast<QueryPattern>(p);
+    return true;
+}
+
+std::vector<Query4D> generateQueries(const VbtFileHeader& header, QueryPattern pattern, uint32_t count, uint32_t seed)
+{
+    const vbt_file_header h = hip_detail::to_c(header);
+    std::vector<uint32_t> rows(4ull * count);
+    if (vbt_generate_queries(&h, static_cast<int>(pattern), count, seed, rows.data()) != VBT_OK)
+        throw std::runtime_error(vbt_last_error());
+    std::vector<Query4D> out(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        out[i].x = rows[4ull * i];
+        out[i].y = rows[4ull * i + 1];
+        out[i].z = rows[4ull * i + 2];
+        out[i].t = rows[4ull * i + 3];
+    }
+    return out;
+}
+
+void fillLeafIndices(const VbtFileHeader& header, std::vector<Query4D>& queries)
+{
+    for (auto& q : queries) q.leafIndex = leafIndexForVoxel(header, q.x, q.y, q.z);
+}
+
+// the key order of vbt_sort_queries_by_leaf, on the record's own leafIndex
+void sortQueriesByLeaf(std::vector<Query4D>& queries)
+{
+    std::stable_sort(queries.begin(), queries.end(), [](const Query4D& a, const Query4D& b) {
+        if (a.leafIndex != b.leafIndex) return a.leafIndex < b.leafIndex;
+        if (a.t != b.t) return a.t < b.t;
+        if (a.z != b.z) return a.z < b.z;
+        if (a.y != b.y) return a.y < b.y;
+        return a.x < b.x;
+    });
 }
 
 } // namespace vbt::render

@@ -210,6 +210,9 @@ def lib():
         L.vbt_batch_noalloc_default_shape.restype = None
         L.vbt_cache_batch_run_noalloc.argtypes = [ctypes.POINTER(CacheJob), c_int, ctypes.POINTER(BatchShape),
                                                   c_void_p]
+        L.vbt_parse_query_pattern.argtypes = [ctypes.c_char_p, ctypes.POINTER(c_int)]
+        L.vbt_generate_queries.argtypes = [ctypes.POINTER(FileHeader), c_int, c_u32, c_u32, c_void_p]
+        L.vbt_sort_queries_by_leaf.argtypes = [ctypes.POINTER(FileHeader), c_void_p, c_u32]
         L.vbt_half_bits_to_float.restype = ctypes.c_float
         L.vbt_half_bits_to_float.argtypes = [ctypes.c_uint16]
         L.vbt_float_to_half_bits.restype = ctypes.c_uint16
@@ -562,6 +565,23 @@ def decode_leaf_values(leaf_bytes: bytes, total_frames, idx, t, use_fine=False, 
 
 
 REQ_QUERY, REQ_CACHE = 0, 1
+
+
+QUERY_PATTERNS = {"random": 0, "same-t": 1, "same-xyz": 2, "coherent": 3}
+
+
+def generate_queries(header: FileHeader, pattern, count: int, seed: int, sort_by_leaf: bool = False) -> np.ndarray:
+    """The reference's request shapes (generateQueries, then sortQueriesByLeaf
+    when sort_by_leaf): a (count, 4) u32 array of (x, y, z, t)."""
+    if isinstance(pattern, str):
+        p = c_int()
+        check(lib().vbt_parse_query_pattern(pattern.encode(), ctypes.byref(p)))
+        pattern = p.value
+    out = np.zeros((count, 4), np.uint32)
+    check(lib().vbt_generate_queries(ctypes.byref(header), int(pattern), count, seed, out.ctypes.data))
+    if sort_by_leaf:
+        check(lib().vbt_sort_queries_by_leaf(ctypes.byref(header), out.ctypes.data, count))
+    return out
 
 
 def default_batch_shape() -> BatchShape:
