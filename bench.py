@@ -338,7 +338,7 @@ def frames_bench(hip, asset, workload: str, steps: int) -> dict:
     """Every frame of one asset into its packed cache in one launch, HIP
     events on the launch stream: the liquid level set (Mode2 shells, the
     sweep kernel with its autotuned tile) and the scientific field (the
-    temporal-DCT leaf kernel, one workgroup per leaf and frame)."""
+    temporal-DCT frame sweep, one workgroup per leaf and run of frames)."""
     info = asset.info
     F = asset.dims[3]
     s = asset.stream
@@ -350,7 +350,7 @@ def frames_bench(hip, asset, workload: str, steps: int) -> dict:
     out.free()
     byt = algorithmic_bytes(info, F)
     shapes = hip.Asset.tile_shapes()
-    kname = (f"k_cache_render<0, {shapes[tile][0]}, {shapes[tile][1]}>" if tile is not None else "k_sci_leaf<false>")
+    kname = (f"k_cache_render<0, {shapes[tile][0]}, {shapes[tile][1]}>" if tile is not None else "k_sci_frames<false>")
     achieved = byt["total"] / (ms * 1e-3) / 1e9
     res = {"workload": f"{workload} {'x'.join(map(str, asset.dims[:3]))} x {F} frames, all frames per launch",
            "kernel": kname, "ms": round(ms, 4), "frames": F, "slots_per_frame": info.controlsPerFrame,

@@ -230,6 +230,7 @@ __global__ void __launch_bounds__(256) k_scan_c(AssetDev a, const Pair* blockBas
                                         ((uint32_t)a.payload[(size_t)d.word * 4u + 2] << 16) |
                                         ((uint32_t)a.payload[(size_t)d.word * 4u + 3] << 24)
                                   : 0u;
+    d.sciEnd = 0xFFFFFFFFu; // set by validation
     d.plan = pl;
     leaves[c] = d;
 }
@@ -285,11 +286,34 @@ __global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, u
     }
     __syncthreads();
     if (a.profile == 2u) {
+        const SciHeader h = sci_header(d.sciHeader);
         if (threadIdx.x == 0) {
-            const SciHeader h = sci_header(d.sciHeader);
             uint32_t st = LEAF_OK;
             if (h.keep > 1u && h.keep - 1u > a.scaleCount) st = LEAF_ERR_SCI_RANGE;
-            if (sci_read_end(a, d) > a.payloadBytes) st = LEAF_ERR_SCI_RANGE;
+            const uint64_t end = sci_read_end(a, d);
+            if (end > a.payloadBytes) st = LEAF_ERR_SCI_RANGE;
+            const uint64_t rel = end - (uint64_t)d.word * 4u;
+            d.sciEnd = rel < 0xFFFFFFFFull ? (uint32_t)rel : 0xFFFFFFFFu;
+            sErr = st;
+        }
+        __syncthreads();
+        // events of one bin in strictly increasing key order: the binary
+        // search of a frame then finds exactly the events a scatter of the
+        // bin finds (the frame sweep's precondition); reads stay inside the
+        // extent checked above
+        if (sErr == LEAF_OK && h.mode == 1u && h.count > 0u) {
+            const Bytes b{a.payload + (size_t)d.word * 4u};
+            const uint32_t binsBase = 4u + 64u * (h.keep + 1u) + 68u, coordsBase = binsBase + 32u;
+            for (uint32_t bin = 0; bin < 16u; ++bin) {
+                const uint32_t start = b.u16(binsBase + bin * 2u);
+                const uint32_t end = bin + 1u < 16u ? b.u16(binsBase + (bin + 1u) * 2u) : (h.count & 0xFFFFu);
+                for (uint32_t i = start + 1u + threadIdx.x; i < end; i += blockDim.x)
+                    if (b.u16(coordsBase + 2u * i) <= b.u16(coordsBase + 2u * (i - 1u))) sMono = 0u;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t st = sErr | (sErr == LEAF_OK && sMono ? FLAG_SWEEP : 0u);
             d.status = st;
             status[d.leaf] = st;
         }
@@ -599,26 +623,18 @@ void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* groupLeaf
 }
 
 // ------------------------------------------------------ scientific decode --
-// scientific leaf-frames: one workgroup per (active leaf, frame), 512 lanes = voxels
-template <bool DENSE_OUT>
-__global__ void __launch_bounds__(512) k_sci_leaf(AssetDev a, int t0, int nt, float* out, uint64_t total,
-                                                  uint32_t* errFlag)
+// scientific dense frame: one workgroup per active leaf, 512 lanes = voxels
+__global__ void __launch_bounds__(512) k_sci_dense(AssetDev a, int t, float* out, uint32_t* errFlag)
 {
     __shared__ float ctrl[64];
     const uint32_t c = blockIdx.x;
-    const int t = t0 + (int)blockIdx.y;
     const DevLeaf& d = a.leaves[c];
     const uint32_t tid = threadIdx.x;
     const uint32_t lx = tid & 7u, ly = (tid >> 3) & 7u, lz = tid >> 6;
     const bool inside = lx < d.lw && ly < d.lh && lz < d.ld;
-    uint64_t dst;
-    if (DENSE_OUT) {
-        uint32_t bx, by, bz, lw, lh, ld;
-        leaf_coords(d.leaf, a.W, a.H, a.D, bx, by, bz, lw, lh, ld);
-        dst = ((uint64_t)(bz * 8u + lz) * a.H + (by * 8u + ly)) * a.W + bx * 8u + lx;
-    } else {
-        dst = (uint64_t)blockIdx.y * total + a.ctrl[c] + (lz * d.lh + ly) * d.lw + lx;
-    }
+    uint32_t bx, by, bz, lw, lh, ld;
+    leaf_coords(d.leaf, a.W, a.H, a.D, bx, by, bz, lw, lh, ld);
+    const uint64_t dst = ((uint64_t)(bz * 8u + lz) * a.H + (by * 8u + ly)) * a.W + bx * 8u + lx;
     if ((d.status & STATUS_MASK) != LEAF_OK) {
         if (inside) out[dst] = __builtin_nanf("");
         if (tid == 0) atomicOr(errFlag, 1u);
@@ -630,6 +646,264 @@ __global__ void __launch_bounds__(512) k_sci_leaf(AssetDev a, int t0, int nt, fl
     __syncthreads();
     if (!inside) return;
     out[dst] = sci_coarse_sample(ctrl, lx, ly, lz) + sci_residual(a, b, h, lx, ly, lz, (uint32_t)t);
+}
+
+// ---------------------------------------------- scientific frame sweep --
+// Packed caches of a run of frames: one workgroup per (active leaf, run of
+// frames).  A leaf whose decode reads at most SCI_STAGE_BYTES (every sparse
+// and empty-residual leaf the encoder writes) is staged into LDS once, a
+// larger one (a dense leaf's per-frame blocks) is read from the payload.  The
+// 64 x keep coefficients (fp16 DC, int8 code x scale table entry) are
+// expanded once; the run is then swept SCI_GROUP frames at a time.  One wave
+// per frame evaluates the 64 controls (the FP64 basis sum of sci_control,
+// term by term in the same order) and carries the coarse trilinear through
+// its x and y passes with lane shuffles: trilinear() lerps x first, then y,
+// then z, so the 4x4x8 x-pass and 4x8x8 y-pass values are exactly the
+// intermediates every voxel of the lattice row would compute.  After one
+// barrier each voxel's thread finishes its z lerp per frame, adds the
+// residual and stores.
+constexpr uint32_t SCI_STAGE_BYTES = 4096;
+constexpr uint32_t SCI_GROUP = 8;      // frames per control pass, one wave each
+constexpr uint32_t SCI_THREADS = 64u * SCI_GROUP; // also one thread per voxel
+constexpr uint64_t SCI_TARGET_GROUPS = 65536; // split runs until the grid holds this many workgroups
+constexpr uint32_t SCI_RUN_MAX = 64;  // frames per workgroup at most (their basis rows sit in LDS)
+
+// a staged leaf: every byte its decode reads is in LDS
+struct LdsBytes {
+    const uint8_t* p;
+    __device__ uint32_t u8(uint32_t o) const { return p[o]; }
+    __device__ uint32_t u16(uint32_t o) const { return (uint32_t)p[o] | ((uint32_t)p[o + 1u] << 8); }
+    __device__ uint32_t u32(uint32_t o) const { return u16(o) | (u16(o + 2u) << 16); }
+    __device__ uint32_t u16a(uint32_t o) const { return *reinterpret_cast<const uint16_t*>(p + o); } // o even
+};
+
+// u16 at an even offset of a leaf (leaves are word-aligned)
+__device__ inline uint32_t u16_even(const LdsBytes& b, uint32_t o) { return b.u16a(o); }
+__device__ inline uint32_t u16_even(const Bytes& b, uint32_t o) { return *reinterpret_cast<const uint16_t*>(b.p + o); }
+
+__device__ inline float lane_float(float v, int src) { return __shfl(v, src, 64); }
+
+// coef: [k][control] for k < keep, then the run's basis rows [frame][k]
+// (dynamic LDS sized by the asset's largest keep): the loop body issues no
+// vector-memory load, whose wait would also wait for the stores before it.
+// yp: per buffer and frame of a group, the y-pass values
+// [cz][y][x] (256 floats).  evt: per voxel, one byte per frame of a group,
+// 0x10 | event code where the frame has an event at the voxel.  yp and evt
+// are double-buffered: a group's waves write one buffer before the barrier
+// that the group's voxel pass reads it after; each voxel's thread clears its
+// evt bytes as it reads them.
+template <bool DENSE, class B>
+__device__ inline void sci_sweep(const AssetDev& a, const B& b, const SciHeader& h, bool sorted, float* coef,
+                                 float (*yp)[SCI_GROUP * 256u], uint64_t (*evt)[512], int t0, int tb, int te,
+                                 float* out, uint64_t total, uint32_t dst, bool in)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lx = tid & 7u, ly = (tid >> 3) & 7u, lz = tid >> 6;
+    for (uint32_t i = tid; i < 64u * h.keep; i += SCI_THREADS) {
+        const uint32_t ci = i & 63u, k = i >> 6;
+        const uint32_t cb = 4u + ci * (h.keep + 1u);
+        coef[i] = k == 0u ? half_to_float(b.u16(cb))
+                          : (float)(int8_t)(uint8_t)b.u8(cb + 2u + (k - 1u)) * a.scales[k - 1u];
+    }
+    float* basisL = coef + 64u * h.keep;
+    for (uint32_t i = tid; i < (uint32_t)(te - tb) * h.keep; i += SCI_THREADS)
+        basisL[i] = a.basis[(size_t)(tb + (int)(i / h.keep)) * 64u + i % h.keep];
+    // sparse residuals: sci_sparse is 0 unless the leaf has events, a positive
+    // event scale and the voxel's mask bit, all fixed over the frames
+    const uint32_t idx = (lz * 8u + ly) * 8u + lx; // == tid
+    const uint32_t sparseBase = 4u + 64u * (h.keep + 1u);
+    const uint32_t binsBase = sparseBase + 68u, coordsBase = binsBase + 32u;
+    const uint32_t residualBase = DENSE ? 0u : coordsBase + sci_event_cap(h) * 2u;
+    float eventScale = 0.0f;
+    bool events = false, act = false;
+    if (!DENSE && h.mode == 1u && h.count != 0u) {
+        eventScale = bits_to_float(b.u32(sparseBase));
+        events = eventScale > 0.0f;
+        if (events) act = ((b.u32(sparseBase + 4u + (idx >> 5) * 4u) >> (idx & 31u)) & 1u) != 0u;
+    }
+    // a sorted leaf's events are scattered per frame group instead of each
+    // voxel searching its frame's bin
+    const bool scatter = events && sorted;
+    if (scatter) {
+        evt[0][tid] = 0u;
+        evt[1][tid] = 0u;
+    }
+    const uint32_t j = tid >> 6, ci = tid & 63u;
+    // the control wave's lane constants: x-pass entries ci and ci + 64
+    // ([cz * 4 + cy][x], x = ci & 7), y-pass entries ci + 64 k ([cz = k][y][x],
+    // y = ci >> 3); sci_coarse_coord gives each lattice cell and weight
+    int xc, yc;
+    float tx, ty;
+    sci_coarse_coord(ci & 7u, xc, tx);
+    sci_coarse_coord(ci >> 3, yc, ty);
+    const int xs0 = (int)(ci >> 3) * 4 + xc;       // control lane of x-pass entry ci (row ci >> 3)
+    const int ys0 = yc * 8 + (int)(ci & 7u);      // x-pass lane of y-pass entry ci, cz even
+    // this voxel's z lerp
+    int zc;
+    float tz;
+    sci_coarse_coord(lz, zc, tz);
+    const uint32_t yo = (uint32_t)zc * 64u + ly * 8u + lx;
+    __syncthreads();
+
+    int g = 0;
+    for (int tg = tb; tg < te; tg += (int)SCI_GROUP, g ^= 1) {
+        const int t = __builtin_amdgcn_readfirstlane(tg + (int)j);
+        if (t < te) {
+            // lane k holds the frame's basis term k; the sum reads it back
+            // lane by lane
+            const int bl = ci < h.keep ? __builtin_bit_cast(int, basisL[(uint32_t)(t - tb) * h.keep + ci]) : 0;
+            double sum = (double)coef[ci] * (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(bl));
+            // a product of two floats is exact in double, so fma rounds as
+            // sci_control's sum + term does
+            uint32_t k = 1;
+            for (; k + 4u <= h.keep; k += 4u) { // four LDS reads in flight per wait
+                float cf[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) cf[u] = coef[(k + u) * 64u + ci];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    sum = __builtin_fma((double)cf[u],
+                                        (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(bl, (int)(k + u))),
+                                        sum);
+            }
+            for (; k < h.keep; ++k)
+                sum = __builtin_fma((double)coef[k * 64u + ci],
+                                    (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(bl, (int)k)), sum);
+            const float cv = (float)sum; // control ci = (cz * 4 + cy) * 4 + cx
+            // x pass (trilinear's c00/c01/c10/c11 for every lattice row)
+            const float x0 = lane_float(cv, xs0) * (1.0f - tx) + lane_float(cv, xs0 + 1) * tx;
+            const float x1 = lane_float(cv, xs0 + 32) * (1.0f - tx) + lane_float(cv, xs0 + 33) * tx;
+            // y pass (its c0/c1): cz 0, 1 from x0's lanes, cz 2, 3 from x1's
+            float* yw = yp[g] + j * 256u;
+            yw[ci] = lane_float(x0, ys0) * (1.0f - ty) + lane_float(x0, ys0 + 8) * ty;
+            yw[ci + 64u] = lane_float(x0, ys0 + 32) * (1.0f - ty) + lane_float(x0, ys0 + 40) * ty;
+            yw[ci + 128u] = lane_float(x1, ys0) * (1.0f - ty) + lane_float(x1, ys0 + 8) * ty;
+            yw[ci + 192u] = lane_float(x1, ys0 + 32) * (1.0f - ty) + lane_float(x1, ys0 + 40) * ty;
+        }
+        const int ng = te - tg < (int)SCI_GROUP ? te - tg : (int)SCI_GROUP;
+        if (scatter && t < te) {
+            // the frame's wave scatters the events of its bin
+            uint8_t* eb = reinterpret_cast<uint8_t*>(evt[g]);
+            const uint32_t bin = sci_event_bin(h, (uint32_t)t);
+            const uint32_t lt = (((uint32_t)t - bin * h.fpb) & 0xFFu) & 0x7Fu; // the key keeps 7 bits of it
+            // binsBase and coordsBase are even
+            const uint32_t start = u16_even(b, binsBase + bin * 2u);
+            const uint32_t end = bin + 1u < 16u ? u16_even(b, binsBase + (bin + 1u) * 2u) : (h.count & 0xFFFFu);
+            for (uint32_t i = start + ci; i < end; i += 64u) {
+                const uint32_t key = u16_even(b, coordsBase + 2u * i);
+                if ((key >> 9) == lt)
+                    eb[(key & 0x1FFu) * 8u + j] = (uint8_t)(0x10 | (sci_event_q(b, residualBase, i) & 0xF));
+            }
+        }
+        __syncthreads(); // the buffers written two groups on are read by no wave still behind this barrier
+        uint64_t row = 0;
+        if (scatter) {
+            row = evt[g][tid];
+            evt[g][tid] = 0u;
+        }
+        if (!in) continue;
+        const float* yr = yp[g] + yo;
+        float* of = out + (uint64_t)(tg - t0) * total; // wave-uniform frame base
+        if (DENSE) {
+            for (int jj = 0; jj < ng; ++jj, of += total) {
+                const float r = sci_dense_value(a, b, h, lx, ly, lz, (uint32_t)(tg + jj));
+                const float v = yr[jj * 256] * (1.0f - tz) + yr[jj * 256 + 64] * tz;
+                __builtin_nontemporal_store(v + r, of + dst);
+            }
+        } else {
+            float r[SCI_GROUP];
+#pragma unroll
+            for (int jj = 0; jj < (int)SCI_GROUP; ++jj) r[jj] = 0.0f;
+            if (scatter) {
+                if (act && row != 0u) {
+#pragma unroll
+                    for (int jj = 0; jj < (int)SCI_GROUP; ++jj) {
+                        const uint32_t e = (uint32_t)(row >> (8 * jj)) & 0xFFu;
+                        int q = (int)(e & 0xFu);
+                        if (q & 8) q -= 16;
+                        r[jj] = e != 0u ? (float)q * eventScale : 0.0f;
+                    }
+                }
+            } else if (act) {
+                for (int jj = 0; jj < ng; ++jj) r[jj] = sci_sparse(b, h, idx, (uint32_t)(tg + jj));
+            }
+#pragma unroll
+            for (int jj = 0; jj < (int)SCI_GROUP; ++jj, of += total) {
+                if (jj >= ng) break;
+                const float v = yr[jj * 256] * (1.0f - tz) + yr[jj * 256 + 64] * tz;
+                __builtin_nontemporal_store(v + r[jj], of + dst);
+            }
+        }
+    }
+}
+
+// DENSE: the leaves with a dense residual (mode 2/3) only, else every other
+// leaf, poisoned ones included; the dense decode's registers would otherwise
+// halve the occupancy of every leaf's sweep.  Dynamic LDS: 4 * (64 +
+// SCI_RUN_MAX) * keepMax bytes of coefficients and basis rows.
+template <bool DENSE>
+__global__ void __launch_bounds__(SCI_THREADS) k_sci_frames(AssetDev a, int t0, int nt, int run, float* out,
+                                                            uint64_t total, uint32_t* errFlag)
+{
+    __shared__ __align__(16) uint8_t stage[SCI_STAGE_BYTES];
+    __shared__ float yp[2][SCI_GROUP * 256u];
+    __shared__ uint64_t evt[2][512];
+    extern __shared__ float coef[];
+    const uint32_t c = blockIdx.x;
+    const int tb = t0 + (int)blockIdx.y * run;
+    const int te = min(t0 + nt, tb + run);
+    const DevLeaf& d = a.leaves[c];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lx = tid & 7u, ly = (tid >> 3) & 7u, lz = tid >> 6;
+    const bool in = lx < d.lw && ly < d.lh && lz < d.ld;
+    // slot of this voxel in a frame: one frame is at most 4 GiB (checked at launch)
+    const uint32_t dst = (uint32_t)(a.ctrl[c] + (lz * d.lh + ly) * d.lw + lx);
+    const bool ok = (d.status & STATUS_MASK) == LEAF_OK;
+    const SciHeader h = sci_header(d.sciHeader);
+    if (DENSE ? !ok || h.mode < 2u : ok && h.mode >= 2u) return;
+    if (!ok) {
+        if (in)
+            for (int t = tb; t < te; ++t) out[(uint64_t)(t - t0) * total + dst] = __builtin_nanf("");
+        if (tid == 0) atomicOr(errFlag, 1u);
+        return;
+    }
+    const uint8_t* leafG = a.payload + (size_t)d.word * 4u;
+    const bool sorted = (d.status & FLAG_SWEEP) != 0u;
+    if (d.sciEnd <= SCI_STAGE_BYTES) {
+        // validation put the read extent inside the payload, whose size is whole words
+        const uint32_t words = (d.sciEnd + 3u) / 4u;
+        for (uint32_t i = tid; i < words; i += SCI_THREADS)
+            reinterpret_cast<uint32_t*>(stage)[i] = reinterpret_cast<const uint32_t*>(leafG)[i];
+        __syncthreads();
+        sci_sweep<DENSE>(a, LdsBytes{stage}, h, sorted, coef, yp, evt, t0, tb, te, out, total, dst, in);
+    } else {
+        sci_sweep<DENSE>(a, Bytes{leafG}, h, sorted, coef, yp, evt, t0, tb, te, out, total, dst, in);
+    }
+}
+
+static void launch_sci_frames(uint32_t A, uint32_t keepMax, bool dense, const AssetDev& ad, int t0, int nt,
+                              float* d_out, uint64_t total, uint32_t* d_err, hipStream_t s)
+{
+    const size_t ldsBytes = 4u * (64u + SCI_RUN_MAX) * (size_t)keepMax;
+    // at most 65535 runs of SCI_RUN_MAX frames per launch
+    const uint64_t span = 65535ull * SCI_RUN_MAX;
+    for (uint64_t f0 = 0; f0 < (uint64_t)nt; f0 += span) {
+        const uint64_t n = std::min<uint64_t>(span, (uint64_t)nt - f0);
+        // runs of whole frame groups: enough of them to fill the chip, none
+        // longer than SCI_RUN_MAX frames
+        uint64_t runs = std::min<uint64_t>((SCI_TARGET_GROUPS + A - 1u) / A, (n + SCI_GROUP - 1u) / SCI_GROUP);
+        runs = std::max<uint64_t>(runs, (n + SCI_RUN_MAX - 1u) / SCI_RUN_MAX);
+        const uint64_t run = std::min<uint64_t>(((n + runs - 1u) / runs + SCI_GROUP - 1u) / SCI_GROUP * SCI_GROUP,
+                                                SCI_RUN_MAX);
+        runs = (n + run - 1u) / run;
+        float* o = d_out + f0 * total;
+        const int ts = t0 + (int)f0;
+        k_sci_frames<false><<<dim3(A, (uint32_t)runs), SCI_THREADS, ldsBytes, s>>>(ad, ts, (int)n, (int)run, o, total,
+                                                                                    d_err);
+        if (dense)
+            k_sci_frames<true><<<dim3(A, (uint32_t)runs), SCI_THREADS, ldsBytes, s>>>(ad, ts, (int)n, (int)run, o,
+                                                                                       total, d_err);
+    }
 }
 
 // ------------------------------------------------------ render voxels --
@@ -894,6 +1168,7 @@ struct vbt_asset_s {
     uint32_t A = 0, bad = 0;
     uint64_t controlsPerFrame = 0, payloadBytes = 0, deviceBytes = 0, fast = 0;
     uint64_t modeCounts[4] = {0, 0, 0, 0};
+    uint32_t sciKeepMax = 1;        // scientific: largest DCT term count of a leaf
     std::string badMessage; // first poisoned leaf and its reference exception text
 
     AssetDev dev() const
@@ -1194,6 +1469,7 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
         uint32_t hw;
         std::memcpy(&hw, base + payOff + 4ull * w, 4);
         a->modeCounts[hw & 3u]++;
+        if (a->h.profileType == 2) a->sciKeepMax = std::max(a->sciKeepMax, ((hw >> 2) & 0x3Fu) + 1u);
         if ((st[l] & STATUS_MASK) != LEAF_OK) {
             if (a->bad == 0)
                 a->badMessage = "first: leaf " + std::to_string(l) + ": " + vbt_status_message((int32_t)(st[l] & STATUS_MASK));
@@ -1268,7 +1544,8 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     const AssetDev ad = a->dev();
     if (a->A == 0) return a->bad ? report_bad(a, "frame cache") : VBT_OK;
     if (a->h.profileType == 2) {
-        k_sci_leaf<false><<<dim3(a->A, (uint32_t)nt), 512, 0, s>>>(ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
+        launch_sci_frames(a->A, a->sciKeepMax, a->modeCounts[2] + a->modeCounts[3] > 0, ad, t0, nt, d_out,
+                          a->controlsPerFrame, a->d_err, s);
     } else if (layout == VBT_CACHE_PACKED) {
         if (nt <= CACHE_EXACT_MAX_FRAMES && a->index.off) {
             launch_indexed(a, a->index, t0, nt, d_out, s);
@@ -1303,7 +1580,7 @@ int vbt_decode_frame_dense(vbt_asset a, int t, const vbt_compose* c, float* d_ou
         return vbt_fail(VBT_ERR_INVALID, "scientific frame index out of range");
     hipStream_t s = S(stream);
     const AssetDev ad = a->dev();
-    if (a->h.profileType == 2) k_sci_leaf<true><<<dim3(a->A, 1), 512, 0, s>>>(ad, t, 1, d_out, 0, a->d_err);
+    if (a->h.profileType == 2) k_sci_dense<<<a->A, 512, 0, s>>>(ad, t, d_out, a->d_err);
     else k_dense_render<<<a->h.leafCount, 512, 0, s>>>(ad, t, to_dev(c), d_out, a->d_err);
     HIPCHK(hipGetLastError());
     if (a->bad) return report_bad(a, "dense frame");

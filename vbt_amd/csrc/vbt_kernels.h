@@ -9,7 +9,8 @@
 
 namespace vbtk {
 
-constexpr uint32_t FLAG_SWEEP = 1u << 8;   // every control's keyframe frames strictly increase
+constexpr uint32_t FLAG_SWEEP = 1u << 8;   // render: every control's keyframe frames strictly increase;
+                                           // scientific: every event bin's keys strictly increase
 constexpr uint32_t FLAG_NO_NAN = 1u << 9;  // no fp16 keyframe value of the leaf is a NaN pattern
 constexpr uint32_t STATUS_MASK = 0xFFu;
 constexpr uint32_t NO_LEAF = 0xffffffffu;
@@ -37,6 +38,7 @@ struct DevLeaf {
     uint32_t ncache, cc;       // cache slots; coarse slots
     uint32_t lw, lh, ld;       // cropped extent
     uint32_t sciHeader;        // scientific header word
+    uint32_t sciEnd;           // scientific: bytes from the leaf start its decode may read (capped)
     LeafPlan plan;
 };
 
@@ -84,7 +86,8 @@ __device__ inline uint32_t find_leaf(const uint64_t* ctrl, uint32_t A, uint64_t 
 }
 
 // ------------------------------------------------------ scientific decode --
-__device__ inline float sci_control(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t ci, int t)
+template <class B>
+__device__ inline float sci_control(const AssetDev& a, const B& b, const SciHeader& h, uint32_t ci, int t)
 {
     const uint32_t cb = 4u + ci * (h.keep + 1u);
     const float* basis = a.basis + (size_t)t * 64u;
@@ -97,7 +100,8 @@ __device__ inline float sci_control(const AssetDev& a, const Bytes& b, const Sci
     return (float)sum;
 }
 
-__device__ inline uint32_t bfp_code(const Bytes& b, uint32_t blockBase, uint32_t ci, uint32_t bits)
+template <class B>
+__device__ inline uint32_t bfp_code(const B& b, uint32_t blockBase, uint32_t ci, uint32_t bits)
 {
     const uint32_t bit = ci * bits;
     const uint32_t byteIndex = blockBase + bit / 8u;
@@ -107,7 +111,8 @@ __device__ inline uint32_t bfp_code(const Bytes& b, uint32_t blockBase, uint32_t
     return (packed >> shift) & ((1u << bits) - 1u);
 }
 
-__device__ inline float sci_dense_value(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t lx, uint32_t ly,
+template <class B>
+__device__ inline float sci_dense_value(const AssetDev& a, const B& b, const SciHeader& h, uint32_t lx, uint32_t ly,
                                         uint32_t lz, uint32_t t)
 {
     const int R = h.mode == 2u ? 3 : 4;
@@ -149,11 +154,32 @@ __device__ inline float sci_dense_value(const AssetDev& a, const Bytes& b, const
     return trilinear(s, tx, ty, tz);
 }
 
-__device__ inline float sci_sparse(const Bytes& b, const SciHeader& h, uint32_t idx, uint32_t t)
+// sparse-event section of a scientific leaf
+// (decodeSparseResidualAt, render/src/scientific_decode.cpp:235-295)
+__device__ inline uint32_t sci_event_cap(const SciHeader& h)
+{
+    return h.tier == 0 ? 64u : h.tier == 1 ? 256u : h.tier == 2 ? 768u : 1024u;
+}
+
+__device__ inline uint32_t sci_event_bin(const SciHeader& h, uint32_t t) { return t / h.fpb < 15u ? t / h.fpb : 15u; }
+
+// the signed 4-bit residual code of event i
+template <class B>
+__device__ inline int sci_event_q(const B& b, uint32_t residualBase, uint32_t i)
+{
+    const uint32_t byte = b.u8(residualBase + (i >> 1));
+    const uint32_t nib = (i & 1u) == 0u ? (byte & 0xFu) : ((byte >> 4) & 0xFu);
+    int q = (int)nib;
+    if (q & 8) q -= 16;
+    return q;
+}
+
+template <class B>
+__device__ inline float sci_sparse(const B& b, const SciHeader& h, uint32_t idx, uint32_t t)
 {
     if (h.count == 0u) return 0.0f;
-    const uint32_t cap = h.tier == 0 ? 64u : h.tier == 1 ? 256u : h.tier == 2 ? 768u : 1024u;
-    const uint32_t bin = t / h.fpb < 15u ? t / h.fpb : 15u;
+    const uint32_t cap = sci_event_cap(h);
+    const uint32_t bin = sci_event_bin(h, t);
     const uint32_t localT = (t - bin * h.fpb) & 0xFFu;
     const uint32_t sparseBase = 4u + 64u * (h.keep + 1u);
     const float eventScale = bits_to_float(b.u32(sparseBase));
@@ -171,20 +197,15 @@ __device__ inline float sci_sparse(const Bytes& b, const SciHeader& h, uint32_t 
     while (left <= right) {
         const int mid = (left + right) >> 1;
         const uint32_t key = b.u16(coordsBase + (uint32_t)mid * 2u);
-        if (key == target) {
-            const uint32_t byte = b.u8(residualBase + (uint32_t)(mid >> 1));
-            const uint32_t nib = (mid & 1) == 0 ? (byte & 0xFu) : ((byte >> 4) & 0xFu);
-            int q = (int)nib;
-            if (q & 8) q -= 16;
-            return (float)q * eventScale;
-        }
+        if (key == target) return (float)sci_event_q(b, residualBase, (uint32_t)mid) * eventScale;
         if (key < target) left = mid + 1;
         else right = mid - 1;
     }
     return 0.0f;
 }
 
-__device__ inline float sci_residual(const AssetDev& a, const Bytes& b, const SciHeader& h, uint32_t lx, uint32_t ly,
+template <class B>
+__device__ inline float sci_residual(const AssetDev& a, const B& b, const SciHeader& h, uint32_t lx, uint32_t ly,
                                      uint32_t lz, uint32_t t)
 {
     if (h.mode == 0u) return 0.0f;
@@ -192,20 +213,36 @@ __device__ inline float sci_residual(const AssetDev& a, const Bytes& b, const Sc
     return sci_dense_value(a, b, h, lx, ly, lz, t);
 }
 
-__device__ inline float sci_coarse_sample(const float* ctrl, uint32_t lx, uint32_t ly, uint32_t lz)
-{
+// the coarse lattice cell and weights of one voxel (sampleCoarseAt,
+// render/src/scientific_decode.cpp:99-123)
+struct SciCoarseAt {
     int x0, y0, z0;
     float tx, ty, tz;
-    sci_coarse_coord(lx, x0, tx);
-    sci_coarse_coord(ly, y0, ty);
-    sci_coarse_coord(lz, z0, tz);
+};
+
+__device__ inline SciCoarseAt sci_coarse_at(uint32_t lx, uint32_t ly, uint32_t lz)
+{
+    SciCoarseAt p;
+    sci_coarse_coord(lx, p.x0, p.tx);
+    sci_coarse_coord(ly, p.y0, p.ty);
+    sci_coarse_coord(lz, p.z0, p.tz);
+    return p;
+}
+
+__device__ inline float sci_coarse_lerp(const float* ctrl, const SciCoarseAt& p)
+{
     float s[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const int cx = x0 + (i & 1), cy = y0 + ((i >> 1) & 1), cz = z0 + ((i >> 2) & 1);
+        const int cx = p.x0 + (i & 1), cy = p.y0 + ((i >> 1) & 1), cz = p.z0 + ((i >> 2) & 1);
         s[i] = ctrl[(cz * 4 + cy) * 4 + cx];
     }
-    return trilinear(s, tx, ty, tz);
+    return trilinear(s, p.tx, p.ty, p.tz);
+}
+
+__device__ inline float sci_coarse_sample(const float* ctrl, uint32_t lx, uint32_t ly, uint32_t lz)
+{
+    return sci_coarse_lerp(ctrl, sci_coarse_at(lx, ly, lz));
 }
 
 // ------------------------------------------------------ render voxels --
