@@ -388,37 +388,143 @@ struct Stream {
     uint32_t keyCount = 0;
 };
 
-void fit_keys(const float* s, int F, float eps, std::vector<int>& keys)
+// Keyframe detector of one series, as the reference's render encoder keys
+// density and control series (detectDensityKeyFrames with its per-frame
+// tolerance, render_temporal_kf_probe_main.cpp:200-275, 2080-2167): start from
+// the two ends, split every span at the frame whose linear-interpolation miss
+// is largest relative to that frame's tolerance until none exceeds it, and
+// keep frames in the render cutoff band and on both sides of every crossing.
+struct KeyProfile {
+    float epsAbs, epsRel, gammaDelta, cutoff, band;
+};
+
+int cutoff_state(float v, const KeyProfile& kp)
+{
+    if (kp.cutoff <= 0.0f) return 1;
+    if (v < kp.cutoff - kp.band) return 0;
+    if (v > kp.cutoff + kp.band) return 2;
+    return 1;
+}
+
+// Frames at which `v` is within `band` of 0 or changes sign, and their
+// neighbours (addIsoSurfaceKeyFrames, render_temporal_kf_probe_main.cpp:2052-2078).
+void mark_iso_keys(const float* v, int F, float band, std::vector<char>& on)
+{
+    band = std::max(0.0f, band);
+    auto mark3 = [&](int t) {
+        for (int u = t - 1; u <= t + 1; ++u)
+            if (u >= 0 && u < F) on[u] = 1;
+    };
+    for (int t = 0; t < F; ++t) {
+        if (std::fabs(v[t]) <= band) mark3(t);
+        if (t > 0 && (v[t - 1] >= 0.0f) != (v[t] >= 0.0f)) {
+            mark3(t - 1);
+            mark3(t);
+        }
+    }
+}
+
+// iso: when set, the iso-surface keys of that series (band isoBand) join the
+// series' own keys, as the reference adds them to level-set voxel and shell
+// series
+void fit_keys(const float* s, int F, const KeyProfile& kp, std::vector<int>& keys, const float* iso = nullptr,
+              float isoBand = 0.0f)
 {
     keys.clear();
-    if (F <= 1) { keys.push_back(0); return; }
-    float mn = s[0], mx = s[0];
-    for (int t = 1; t < F; ++t) { mn = std::min(mn, s[t]); mx = std::max(mx, s[t]); }
-    if (mx == mn) { keys.push_back(0); return; }
+    if (F <= 0) return;
+    if (F == 1) {
+        keys.push_back(0);
+        return;
+    }
     std::vector<char> on(F, 0);
+    auto finish = [&]() {
+        if (iso) mark_iso_keys(iso, F, isoBand, on);
+        for (int t = 0; t < F; ++t)
+            if (on[t]) keys.push_back(t);
+    };
+    double sum = 0.0, sum2 = 0.0;
+    for (int t = 0; t < F; ++t) {
+        sum += s[t];
+        sum2 += (double)s[t] * s[t];
+    }
+    const float floorEps = std::max(1e-6f, kp.epsAbs * 0.1f);
+    const double dmean = sum / F;
+    if (std::max(0.0, sum2 / F - dmean * dmean) < (double)floorEps * floorEps) { // constant: one key
+        on[0] = 1;
+        finish();
+        return;
+    }
+    const float mean = (float)dmean;
+    const float sd = (float)std::sqrt(std::max(0.0, sum2 / F - (double)mean * mean));
+    std::vector<float> steps(F - 1);
+    for (int t = 1; t < F; ++t) steps[t - 1] = std::fabs(s[t] - s[t - 1]);
+    std::sort(steps.begin(), steps.end());
+    const float stepScale = std::max(1e-6f, steps[std::min(F - 2, (F - 1) * 9 / 10)]);
+    std::vector<float> tol(F);
+    for (int t = 0; t < F; ++t) {
+        float e = kp.epsAbs + kp.epsRel * std::max(std::fabs(s[t]), sd);
+        if (kp.gammaDelta > 0.0f && t > 0)
+            e /= 1.0f + kp.gammaDelta * (std::fabs(s[t] - s[t - 1]) / stepScale);
+        tol[t] = std::max(e, floorEps);
+    }
     on[0] = on[F - 1] = 1;
     std::vector<std::pair<int, int>> stack{{0, F - 1}};
-    int count = 2;
-    while (!stack.empty() && count < 255) {
+    while (!stack.empty()) {
         auto [a, b] = stack.back();
         stack.pop_back();
         if (b - a < 2) continue;
-        float worst = 0.0f;
+        double worst = 0.0;
         int wt = -1;
+        const double v0 = s[a], v1 = s[b];
         for (int t = a + 1; t < b; ++t) {
-            const float al = (float)(t - a) / (float)(b - a);
-            const float e = std::fabs(s[t] - (s[a] + al * (s[b] - s[a])));
-            if (e > worst) { worst = e; wt = t; }
+            const double pred = v0 + (double)(t - a) / (double)(b - a) * (v1 - v0);
+            const double e = std::fabs((double)s[t] - pred) / (double)tol[t];
+            if (e > worst) {
+                worst = e;
+                wt = t;
+            }
         }
-        if (wt >= 0 && worst > eps) {
+        if (wt >= 0 && worst > 1.0) {
             on[wt] = 1;
-            ++count;
             stack.push_back({a, wt});
             stack.push_back({wt, b});
         }
     }
-    for (int t = 0; t < F; ++t)
-        if (on[t]) keys.push_back(t);
+    if (kp.cutoff > 0.0f && kp.band >= 0.0f) {
+        auto mark = [&](int t) {
+            if (t >= 0 && t < F) on[t] = 1;
+        };
+        for (int t = 0; t < F; ++t) {
+            const int st = cutoff_state(s[t], kp);
+            if (st == 1 || (t > 0 && cutoff_state(s[t - 1], kp) != st)) {
+                mark(t - 1);
+                mark(t);
+                mark(t + 1);
+            }
+        }
+    }
+    finish();
+}
+
+// The series its keys interpolate (reconstructFromKeys,
+// render_temporal_kf_probe_main.cpp:2169-2200): exact values at the keys,
+// linear between them, held before the first and after the last.
+void recon_linear(const float* s, const std::vector<int>& keys, int F, float* out)
+{
+    if (keys.empty()) {
+        std::copy(s, s + F, out);
+        return;
+    }
+    size_t seg = 0;
+    for (int t = 0; t < F; ++t) {
+        while (seg + 1 < keys.size() && t > keys[seg + 1]) ++seg;
+        if (seg + 1 >= keys.size()) {
+            out[t] = s[keys.back()];
+            continue;
+        }
+        const int a = keys[seg], b = keys[seg + 1];
+        out[t] = t <= a ? s[a] : (float)((double)s[a] + (double)(t - a) / (double)(b - a) * (double)(s[b] - s[a]));
+    }
 }
 
 // bins: buildBinIndexFromKeys semantics (last key before each bin start)
@@ -531,12 +637,6 @@ void controls_for(const FieldLike& fd, int res, int x0, int y0, int z0, int F, s
     }
 }
 
-float series_eps(const float* s, int F, const vbts_encode_opts& o)
-{
-    float mn = s[0], mx = s[0];
-    for (int t = 1; t < F; ++t) { mn = std::min(mn, s[t]); mx = std::max(mx, s[t]); }
-    return std::max(o.epsAbs, o.epsRel * (mx - mn));
-}
 
 // The series a control's keyframes decode to: the first key before it, the
 // last after it, v0 + a (v1 - v0) between (fp16 keys rounded as stored).
@@ -564,7 +664,8 @@ void build_stream(const std::vector<float>& series, int C, int F, int B, int cod
     std::vector<int> keys;
     for (int c = 0; c < C; ++c) {
         const float* s = &series[(size_t)c * F];
-        fit_keys(s, F, series_eps(s, F, o) * epsScale, keys);
+        fit_keys(s, F, KeyProfile{o.epsAbs * epsScale, o.epsRel * epsScale, o.gammaDelta, o.cutoff, o.cutoffBand},
+                 keys);
         if (o.burstFraction > 0.0f && o.burstMax > 1) {
             const uint64_t h = mix64(seed ^ ((uint64_t)c * 0x9E3779B97F4A7C15ull));
             if (hash01(h) < (double)o.burstFraction) {
@@ -586,7 +687,140 @@ void build_stream(const std::vector<float>& series, int C, int F, int B, int cod
     }
 }
 
-int encode_render_leaf(const Field& full, const vbts_encode_opts& o, uint32_t leaf, std::vector<uint8_t>& out)
+// The range of a render field's values over its voxels and frames, which the
+// route's errors are relative to (the reference divides by the raw data's
+// max - min).  Density: the largest value at the voxels around every blob
+// centre at every frame (the field is a sum of blobs and 0 far from them);
+// flames peak at 1 (the band of the density); temperature spans 300 K to
+// 1800 K.
+double field_range(const Field& f)
+{
+    const vbts_params& p = f.p;
+    if (p.kind == VBTS_FLAMES) return 1.0;
+    if (p.kind == VBTS_TEMPERATURE) return 1500.0;
+    if (p.kind != VBTS_DENSITY) return 1.0;
+    double mx = 0.0;
+    for (uint32_t t = 0; t < p.frames; ++t)
+        for (size_t i = 0; i < f.blobs.size(); ++i) {
+            double tmp[3];
+            const double* c = f.centre(i, (double)t, tmp);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const double x = std::floor(c[0] + 0.5) + dx, y = std::floor(c[1] + 0.5) + dy,
+                                     z = std::floor(c[2] + 0.5) + dz;
+                        if (x < 0 || y < 0 || z < 0 || x >= p.width || y >= p.height || z >= p.depth) continue;
+                        mx = std::max(mx, f.eval(x, y, z, (double)t, true));
+                    }
+        }
+    return std::max(1e-6, mx);
+}
+
+// leaf bytes: header, prefix (u16 keyframe counts), coarse stream, then the
+// Fine6 stream or the Mode2 shell section (mask, group prefix, shell stream)
+int emit_render_leaf(std::vector<uint8_t>& out, const vbts_encode_opts& o, int mode, Stream& cs, Stream& fs,
+                     const std::vector<uint16_t>& shellVox)
+{
+    const uint32_t header = (uint32_t)mode | ((uint32_t)o.codec << 2) | ((uint32_t)o.codec << 4) | (4u << 6) |
+                            ((mode == 3 ? 6u : 0u) << 9);
+    put32(out, header);
+    put16(out, cs.keyCount);
+    put16(out, mode == 1 ? 0u : fs.keyCount);
+    emit_stream(out, cs);
+    if (mode == 3) emit_stream(out, fs);
+    if (mode == 2) {
+        uint64_t mask[8] = {0};
+        for (uint16_t v : shellVox) mask[v >> 6] |= 1ull << (v & 63);
+        for (int gI = 0; gI < 8; ++gI)
+            for (int b = 0; b < 8; ++b) out.push_back((uint8_t)(mask[gI] >> (8 * b)));
+        uint32_t prefix = 0;
+        for (int gI = 0; gI < 8; ++gI) {
+            put16(out, prefix);
+            prefix += (uint32_t)__builtin_popcountll(mask[gI]);
+        }
+        emit_stream(out, fs);
+    }
+    return 0;
+}
+
+// Fine6 or Grid4 for a non-empty leaf of a blob field, by the reference's
+// route rule (routeRenderTemporalLeaf, src/render_temporal_route.cpp:27-66, on
+// metrics taken as render_temporal_kf_probe_main.cpp:1185-1240 takes them):
+// at every voxel and frame of the leaf whose coordinates are multiples of 4,
+// the field against the decoded coarse grid and against coarse + decoded 6^3
+// residual, errors relative to the field's range.
+bool route_fine6(const LeafField& fd, const vbts_encode_opts& o, double range, int x0, int y0, int z0, int lw,
+                 int lh, int ld, int F, const std::vector<float>& recCoarse, const std::vector<float>& recFine)
+{
+    double ce2 = 0.0, fe2 = 0.0, cpk = 0.0;
+    uint64_t n = 0, visible = 0, inBand = 0;
+    const double lo = (double)o.cutoff - o.cutoffBand, hi = (double)o.cutoff + o.cutoffBand;
+    std::vector<float> g(64), g6(216);
+    for (int t = 0; t < F; t += 4) {
+        for (int c = 0; c < 64; ++c) g[c] = recCoarse[(size_t)c * F + t];
+        for (int c = 0; c < 216; ++c) g6[c] = recFine[(size_t)c * F + t];
+        for (int z = (4 - z0 % 4) % 4; z < ld; z += 4)
+            for (int y = (4 - y0 % 4) % 4; y < lh; y += 4)
+                for (int x = (4 - x0 % 4) % 4; x < lw; x += 4) {
+                    const double truth = fd.eval(x0 + x, y0 + y, z0 + z, t, true);
+                    const double cp = trilinear_lattice(g.data(), 4, x, y, z);
+                    const double fp = cp + trilinear_lattice(g6.data(), 6, x, y, z);
+                    ++n;
+                    visible += truth >= lo;
+                    inBand += truth >= lo && truth <= hi;
+                    ce2 += (truth - cp) * (truth - cp);
+                    fe2 += (truth - fp) * (truth - fp);
+                    cpk = std::max(cpk, std::fabs(truth - cp) / range);
+                }
+    }
+    if (n == 0) return false;
+    const double cr = std::sqrt(ce2 / n) / range, fr = std::sqrt(fe2 / n) / range;
+    const bool rough = cr > o.routeCoarseRmse || cpk > o.routeCoarsePeak;
+    const bool seen = (double)visible / n >= o.routeVisible || (double)inBand / n >= o.routeBand;
+    const double gain = (cr - fr) / std::max(cr, 1e-12);
+    return rough && seen && gain >= o.routeFineGain;
+}
+
+// A stream's keyframe count is u16 in the leaf prefix: its tolerance grows by
+// 1.25x until it fits.
+int build_stream_fit(const std::vector<float>& series, int C, int F, int B, const vbts_encode_opts& o, uint64_t seed,
+                     Stream& st, std::vector<float>* rec)
+{
+    for (float sc = 1.0f;; sc *= 1.25f) {
+        st = Stream{};
+        build_stream(series, C, F, B, o.codec, sc, o, seed, st, rec);
+        if (st.keyCount <= 65535u) return 0;
+        if (sc > 1e6f) {
+            g_err = "leaf keyframe count exceeds u16";
+            return 1;
+        }
+    }
+}
+
+// The Mode2 shell stream: each shell voxel's residual keyed with the iso keys
+// of its series; the tolerance grows by 1.25x until the keyframes fit u16.
+int build_shell_stream(const std::vector<float>& resid, const std::vector<float>& truth, int C, int F, float isoBand,
+                       const vbts_encode_opts& o, uint64_t seed, Stream& st)
+{
+    (void)seed;
+    std::vector<int> keys;
+    for (float sc = 1.0f;; sc *= 1.25f) {
+        st = Stream{};
+        const KeyProfile kp{o.epsAbs * sc, o.epsRel * sc, o.gammaDelta, o.cutoff, o.cutoffBand};
+        for (int c = 0; c < C; ++c) {
+            fit_keys(&resid[(size_t)c * F], F, kp, keys, &truth[(size_t)c * F], isoBand);
+            append_control(st, &resid[(size_t)c * F], keys, F, 4, o.codec);
+        }
+        if (st.keyCount <= 65535u) return 0;
+        if (sc > 1e6f) {
+            g_err = "leaf keyframe count exceeds u16";
+            return 1;
+        }
+    }
+}
+
+int encode_render_leaf(const Field& full, const vbts_encode_opts& o, double range, uint32_t leaf,
+                       std::vector<uint8_t>& out)
 {
     const vbts_params& p = full.p;
     const int lcx = (int)((p.width + 7) / 8), lcy = (int)((p.height + 7) / 8);
@@ -619,74 +853,97 @@ int encode_render_leaf(const Field& full, const vbts_encode_opts& o, uint32_t le
             return 0;
         }
     }
+    const uint64_t seed = mix64(p.seed ^ ((uint64_t)leaf * 0x2545F4914F6CDD1Dull));
+    // The fine / shell residuals are taken against the coarse grid as it
+    // decodes (closed loop), so the coarse keyframe error does not add to theirs.
+    Stream cs, fs;
+    std::vector<float> recCoarse;
+    if (build_stream_fit(coarse, 64, F, 8, o, seed, cs, &recCoarse)) return 1;
     int mode = 1; // GRID4
-    std::vector<uint16_t> shellVox;
     if (p.kind == VBTS_LEVELSET) {
-        double minAbs = 1e30;
-        for (int t = 0; t < F; ++t)
-            minAbs = std::min(minAbs, std::fabs(fd.eval(x0 + 3.5, y0 + 3.5, z0 + 3.5, t, false)));
-        if (minAbs < (double)o.shellBand + 6.1) {
+        // Mode2 as the reference's level-set surface route builds it
+        // (render_temporal_kf_probe_main.cpp:600-660, 940-1010): each voxel's
+        // series keyed with iso-surface protection and replaced by what its
+        // keys interpolate; a voxel is a shell voxel where that series comes
+        // within the surface band of 0, the decoded coarse grid within the
+        // guard band, or the two differ in sign, at some frame; its series is
+        // the residual to the coarse grid, keyed with its iso keys
+        const float surface = o.shellBand, guard = std::max(0.5f, 0.5f * o.shellBand), isoBand = 0.1f;
+        // a voxel whose values all stay below this is background (its series
+        // becomes 0), a leaf of such voxels is empty (bgZeroRatio 0.3 of the cutoff)
+        const float bg = 0.3f * o.cutoff;
+        const KeyProfile vp{o.epsAbs, o.epsRel, o.gammaDelta, o.cutoff, o.cutoffBand};
+        std::vector<uint16_t> shellVox;
+        std::vector<float> truth, resid;
+        double minAbs = 1e30, maxCentre = -1e30;
+        for (int t = 0; t < F; ++t) {
+            const double c = fd.eval(x0 + 3.5, y0 + 3.5, z0 + 3.5, t, false);
+            minAbs = std::min(minAbs, std::fabs(c));
+            maxCentre = std::max(maxCentre, c);
+        }
+        // phi changes by about a voxel per voxel: a centre this far inside at
+        // every frame leaves every voxel of the leaf below the background
+        if (maxCentre < -((double)surface + 6.1)) {
+            out.clear();
+            put32(out, 0u);
+            return 0;
+        }
+        if (minAbs < (double)surface + 6.1) {
+            float leafMax = -1e30f;
+            std::vector<float> g(64), raw(F), rec(F), tr((size_t)512 * F), cg((size_t)512 * F);
+            std::vector<int> keys;
+            for (int t = 0; t < F; ++t) {
+                for (int c = 0; c < 64; ++c) g[c] = recCoarse[(size_t)c * F + t];
+                for (int v = 0; v < 512; ++v)
+                    cg[(size_t)v * F + t] = trilinear_lattice(g.data(), 4, v & 7, (v >> 3) & 7, v >> 6);
+            }
             for (int z = 0; z < ld; ++z)
                 for (int y = 0; y < lh; ++y)
                     for (int x = 0; x < lw; ++x) {
+                        const int v = (z * 8 + y) * 8 + x;
+                        float vmax = -1e30f;
+                        for (int t = 0; t < F; ++t) {
+                            raw[t] = (float)fd.eval(x0 + x, y0 + y, z0 + z, t, true);
+                            vmax = std::max(vmax, raw[t]);
+                        }
+                        if (vmax < bg) {
+                            std::fill(rec.begin(), rec.end(), 0.0f);
+                        } else {
+                            fit_keys(raw.data(), F, vp, keys, raw.data(), isoBand);
+                            recon_linear(raw.data(), keys, F, rec.data());
+                        }
+                        for (int t = 0; t < F; ++t) leafMax = std::max(leafMax, rec[t]);
+                        const float* c = &cg[(size_t)v * F];
                         bool occ = false;
                         for (int t = 0; t < F && !occ; ++t)
-                            if (std::fabs(fd.eval(x0 + x, y0 + y, z0 + z, t, true)) < o.shellBand) occ = true;
-                        if (occ) shellVox.push_back((uint16_t)((z * 8 + y) * 8 + x));
+                            occ = std::fabs(rec[t]) <= surface || std::fabs(c[t]) <= guard || (rec[t] >= 0.0f) != (c[t] >= 0.0f);
+                        if (!occ) continue;
+                        shellVox.push_back((uint16_t)v);
+                        std::copy(rec.begin(), rec.end(), tr.begin() + (size_t)v * F);
                     }
-            std::sort(shellVox.begin(), shellVox.end());
-            if (!shellVox.empty()) mode = 2;
-        }
-    } else {
-        // route probe: coarse-grid miss on the 6^3 lattice every 4th frame
-        std::vector<float> g(64);
-        float worst = 0.0f;
-        const double sc6 = 7.0 / 5.0;
-        for (int t = 0; t < F; t += 4) {
-            for (int c = 0; c < 64; ++c) g[c] = coarse[(size_t)c * F + t];
-            for (int j = 0; j < 216; ++j) {
-                const double x = (j % 6) * sc6, y = ((j / 6) % 6) * sc6, z = (j / 36) * sc6;
-                const float r = (float)fd.eval(x0 + x, y0 + y, z0 + z, t, false) - trilinear_lattice(g.data(), 4, x, y, z);
-                worst = std::max(worst, std::fabs(r));
+            if (leafMax < bg) {
+                out.clear();
+                put32(out, 0u);
+                return 0;
+            }
+            if (!shellVox.empty()) {
+                mode = 2;
+                resid.resize(shellVox.size() * (size_t)F);
+                truth.resize(shellVox.size() * (size_t)F);
+                for (size_t i = 0; i < shellVox.size(); ++i)
+                    for (int t = 0; t < F; ++t) {
+                        const size_t vt = (size_t)shellVox[i] * F + t;
+                        truth[i * F + t] = tr[vt];
+                        resid[i * F + t] = tr[vt] - cg[vt];
+                    }
             }
         }
-        const float scale = p.kind == VBTS_TEMPERATURE ? 1500.0f : 1.0f;
-        if (worst > o.fineThreshold * scale) mode = 3;
+        if (mode == 2 && build_shell_stream(resid, truth, (int)shellVox.size(), F, isoBand, o, seed ^ 2u, fs)) return 1;
+        return emit_render_leaf(out, o, mode, cs, fs, shellVox);
     }
-    const uint64_t seed = mix64(p.seed ^ ((uint64_t)leaf * 0x2545F4914F6CDD1Dull));
-    // Each stream is rate-controlled on its own (the leaf prefix holds u16
-    // keyframe counts): its error bound grows by 1.25x until it fits.  The
-    // fine / shell residuals are taken against the coarse grid as it decodes
-    // (closed loop), so the coarse keyframe error does not add to theirs.
-    Stream cs, fs;
-    std::vector<float> recCoarse;
-    for (float sc = 1.0f;; sc *= 1.25f) {
-        cs = Stream{};
-        build_stream(coarse, 64, F, 8, o.codec, sc, o, seed, cs, &recCoarse);
-        if (cs.keyCount <= 65535u) break;
-        if (sc > 1e6f) {
-            g_err = "leaf keyframe count exceeds u16";
-            return 1;
-        }
-    }
-    std::vector<float> resid;
-    int residCount = 0;
-    if (mode == 2) {
-        residCount = (int)shellVox.size();
-        resid.resize(shellVox.size() * (size_t)F);
-        std::vector<float> g(64);
-        for (int t = 0; t < F; ++t) {
-            for (int c = 0; c < 64; ++c) g[c] = recCoarse[(size_t)c * F + t];
-            for (size_t i = 0; i < shellVox.size(); ++i) {
-                const int v = shellVox[i];
-                const int x = v & 7, y = (v >> 3) & 7, z = v >> 6;
-                resid[i * F + t] =
-                    (float)(fd.eval(x0 + x, y0 + y, z0 + z, t, true) - trilinear_lattice(g.data(), 4, x, y, z));
-            }
-        }
-    } else if (mode == 3) {
-        residCount = 216;
-        resid.resize((size_t)216 * F);
+    // the 6^3 residual of the decoded coarse grid, kept if the route picks Fine6
+    std::vector<float> resid((size_t)216 * F), recFine;
+    {
         std::vector<float> g(64);
         const double sc6 = 7.0 / 5.0;
         for (int t = 0; t < F; ++t) {
@@ -698,39 +955,9 @@ int encode_render_leaf(const Field& full, const vbts_encode_opts& o, uint32_t le
             }
         }
     }
-    if (mode != 1) {
-        for (float sc = 1.0f;; sc *= 1.25f) {
-            fs = Stream{};
-            build_stream(resid, residCount, F, mode == 2 ? 4 : 8, o.codec, sc, o, seed ^ (mode == 2 ? 2u : 1u), fs);
-            if (fs.keyCount <= 65535u) break;
-            if (sc > 1e6f) {
-                g_err = "leaf keyframe count exceeds u16";
-                return 1;
-            }
-        }
-    }
-    {
-        const uint32_t header = (uint32_t)mode | ((uint32_t)o.codec << 2) | ((uint32_t)o.codec << 4) | (4u << 6) |
-                                ((mode == 3 ? 6u : 0u) << 9);
-        put32(out, header);
-        put16(out, cs.keyCount);
-        put16(out, fs.keyCount);
-        emit_stream(out, cs);
-        if (mode == 3) emit_stream(out, fs);
-        if (mode == 2) {
-            uint64_t mask[8] = {0};
-            for (uint16_t v : shellVox) mask[v >> 6] |= 1ull << (v & 63);
-            for (int gI = 0; gI < 8; ++gI)
-                for (int b = 0; b < 8; ++b) out.push_back((uint8_t)(mask[gI] >> (8 * b)));
-            uint32_t prefix = 0;
-            for (int gI = 0; gI < 8; ++gI) {
-                put16(out, prefix);
-                prefix += (uint32_t)__builtin_popcountll(mask[gI]);
-            }
-            emit_stream(out, fs);
-        }
-        return 0;
-    }
+    if (build_stream_fit(resid, 216, F, 8, o, seed ^ 1u, fs, &recFine)) return 1;
+    if (route_fine6(fd, o, range, x0, y0, z0, lw, lh, ld, F, recCoarse, recFine)) mode = 3;
+    return emit_render_leaf(out, o, mode, cs, fs, {});
 }
 
 // ---------------------------------------------------------- scientific --
@@ -1026,10 +1253,19 @@ void vbts_default_encode_opts(vbts_encode_opts* o)
 {
     std::memset(o, 0, sizeof(*o));
     o->codec = 0;
-    o->epsAbs = 1e-4f;
+    // the reference render encoder's defaults (ProbeCliOptions,
+    // render_temporal_kf_probe_main.cpp:37-71)
+    o->epsAbs = 1e-5f;
     o->epsRel = 0.02f;
-    o->fineThreshold = 0.02f;
-    o->shellBand = 2.0f;
+    o->gammaDelta = 0.2f;
+    o->cutoff = 0.0003f;
+    o->cutoffBand = 0.0001f;
+    o->routeCoarseRmse = 0.015f;
+    o->routeCoarsePeak = 0.06f;
+    o->routeVisible = 0.02f;
+    o->routeBand = 0.01f;
+    o->routeFineGain = 0.15f;
+    o->shellBand = 1.5f; // the reference's level-set surface band without shell_width_voxels
     o->burstFraction = 0.0f;
     o->burstMax = 32;
     o->sciKeep = 12;
@@ -1054,6 +1290,7 @@ int vbts_encode(void* field, const vbts_encode_opts* o, uint8_t** outBytes, uint
         }
         const uint32_t lcx = (p.width + 7) / 8, lcy = (p.height + 7) / 8, lcz = (p.depth + 7) / 8;
         const uint32_t L = lcx * lcy * lcz;
+        const double range = field_range(fd);
         std::vector<std::vector<uint8_t>> leaves(L);
         std::atomic<int> err{0};
         std::string msg;
@@ -1061,7 +1298,7 @@ int vbts_encode(void* field, const vbts_encode_opts* o, uint8_t** outBytes, uint
 #pragma omp parallel for schedule(dynamic, 4) num_threads(o->threads > 0 ? o->threads : 1)
         for (int64_t l = 0; l < (int64_t)L; ++l) {
             if (err.load()) continue;
-            if (encode_render_leaf(fd, *o, (uint32_t)l, leaves[l])) {
+            if (encode_render_leaf(fd, *o, range, (uint32_t)l, leaves[l])) {
                 std::lock_guard<std::mutex> g(mu);
                 err = 1;
                 msg = g_err;

@@ -42,12 +42,23 @@ void vbts_field_free(void* field);
 int vbts_field_frame(void* field, uint32_t t, float* out, int threads);
 float vbts_field_eval(void* field, float x, float y, float z, uint32_t t);
 
+// Render fields are encoded the way the reference's render encoder does it
+// (src/render_temporal_kf_probe_main.cpp): keyframes by the adaptive
+// piecewise-linear split of its density detector with cutoff protection, and
+// each non-empty leaf routed to Fine6 or Grid4 by its sampled coarse error.
 typedef struct {
     int codec;            // 0 fp16 keyframes, 1 fp32
-    float epsAbs;         // keyframe fit tolerance: max(epsAbs, epsRel * series range)
-    float epsRel;
-    float fineThreshold;  // route to Fine6 when the coarse grid misses by more than this
-    float shellBand;      // Mode2: |phi| < band marks shell voxels (levelset fields)
+    float epsAbs;         // keyframe tolerance at frame t: epsAbs + epsRel * max(|v_t|, stddev),
+    float epsRel;         //   divided by 1 + gammaDelta * |v_t - v_t-1| / (90th percentile step)
+    float gammaDelta;
+    float cutoff;         // render cutoff: frames in [cutoff - band, cutoff + band] or crossing it keep keys
+    float cutoffBand;
+    float routeCoarseRmse; // Fine6 when the coarse error (relative to the field's range) exceeds either
+    float routeCoarsePeak; //   of these, the leaf is visible enough and the 6^3 residual gains enough
+    float routeVisible;   // fraction of samples >= cutoff - band
+    float routeBand;      // fraction of samples inside the cutoff band
+    float routeFineGain;  // (coarse RMSE - Fine6 RMSE) / coarse RMSE
+    float shellBand;      // Mode2 surface band: shell voxels come within it of phi = 0 (guard band: half of it, at least 0.5)
     float burstFraction;  // fraction of controls that receive extra burst keyframes
     uint32_t burstMax;    // burst keyframes per control 1..burstMax-1 extra
     int sciKeep;          // scientific: DCT terms kept per control

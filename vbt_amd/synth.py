@@ -45,7 +45,14 @@ class EncodeOpts(ctypes.Structure):
         ("codec", ctypes.c_int),
         ("epsAbs", ctypes.c_float),
         ("epsRel", ctypes.c_float),
-        ("fineThreshold", ctypes.c_float),
+        ("gammaDelta", ctypes.c_float),
+        ("cutoff", ctypes.c_float),
+        ("cutoffBand", ctypes.c_float),
+        ("routeCoarseRmse", ctypes.c_float),
+        ("routeCoarsePeak", ctypes.c_float),
+        ("routeVisible", ctypes.c_float),
+        ("routeBand", ctypes.c_float),
+        ("routeFineGain", ctypes.c_float),
         ("shellBand", ctypes.c_float),
         ("burstFraction", ctypes.c_float),
         ("burstMax", ctypes.c_uint32),
