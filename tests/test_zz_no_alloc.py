@@ -126,12 +126,13 @@ def test_limits_are_rejected(assets):
 
 
 def test_tile_limit_is_rejected():
-    """More leaf tiles than the scratch array holds: 28 one-frame jobs over
-    smoke256 at 64 slots per tile (28 x 19,083 tiles > 2^19).  The outputs
-    are full size, so even a run that went ahead would stay in bounds."""
+    """More leaf tiles than the scratch array holds: one-frame jobs over
+    fire_temperature at 64 slots per tile (7 x 79,571 tiles > 2^19).  The
+    outputs are full size, so even a run that went ahead would stay in
+    bounds."""
     from workloads import encoded
 
-    a = hip.Asset(encoded("smoke256"))
+    a = hip.Asset(encoded("fire_temperature"))
     tiles = (a.info.controlsPerFrame + 63) // 64
     n = (1 << 19) // tiles + 1
     assert n <= 32
@@ -146,7 +147,7 @@ def test_tile_limit_is_rejected():
         hip.cache_batch_run_noalloc([(a, 7 * i, 1, hip.CACHE_PACKED, o) for i, o in enumerate(sel)], shape(tile=0), s)
         s.sync()
         for i, o in enumerate(sel):
-            bitexact(o.host()[0], a.frame_cache(7 * i, 1)[0], f"smoke256 frame {7 * i}")
+            bitexact(o.host()[0], a.frame_cache(7 * i, 1)[0], f"fire_temperature frame {7 * i}")
     for o in outs:
         o.free()
     a.close()
