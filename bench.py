@@ -344,7 +344,7 @@ def frames_bench(hip, asset, workload: str, steps: int) -> dict:
     s = asset.stream
     tile = None
     if info.header.profileType != 2:
-        tile = asset.autotune(F, reps=1)[0]
+        tile = asset.autotune_table(F, reps=1)[0]
     out = hip.DeviceArray((F, info.controlsPerFrame), np.float32, s)
     ms = timed(hip, s, lambda: asset.frame_cache_into(out, 0, F), max(3, steps // 2), 1)
     out.free()
@@ -378,7 +378,7 @@ def batch_bench(hip, assets: dict, names, steps: int, tune_reps: int = 2) -> dic
             byt[k] += v
     tiles = {}
     for a, t0, nt, layout, out in jobs:
-        tiles[a] = a.autotune(nt, reps=1)[0]
+        tiles[a] = a.autotune_table(nt, reps=1)[0]
 
     def plain():
         for a, t0, nt, layout, out in jobs:
@@ -654,7 +654,12 @@ def main():
     ap.add_argument("--batch-tune-reps", type=int, default=2)
     ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
                     help="assets of the batched frame-cache build (all frames of each, one graph replay)")
-    ap.add_argument("--tile", type=int, default=-1, help="frame-cache tile shape index (default: autotune)")
+    ap.add_argument("--tile", type=int, default=-1, help="frame-cache tile shape index (default: tuning table, "
+                    "else autotune)")
+    ap.add_argument("--tuning", default=os.path.join(ROOT, "vbt_amd", "tuning_gfx950.json"),
+                    help="tuning table to start from (missing file: start empty)")
+    ap.add_argument("--retune", action="store_true", help="start from an empty tuning table")
+    ap.add_argument("--save-tuning", default="", help="write the tuning table here at the end")
     ap.add_argument("--single-frame-index", action="store_true",
                     help="build the frame index before the single-frame probe")
     ap.add_argument("--single-frame-probe", type=int, default=0,
@@ -672,6 +677,8 @@ def main():
     if hip.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if not args.retune and args.tuning and os.path.exists(args.tuning):
+        hip.tuning_load(args.tuning)
     w = synth.WORKLOADS[args.workload]
     HT = host_cpu()["threads"]
     t0 = time.time()
@@ -685,12 +692,17 @@ def main():
     W, H, D, F = asset.dims
     nt = F
     shapes = hip.Asset.tile_shapes()
+    tile_source = "fixed"
     if args.tile >= 0:
         asset.cache_tile = args.tile
         best, tile_ms = args.tile, [None] * len(shapes)
     else:
-        best, tile_ms = asset.autotune(nt, reps=2)
-    print(f"[bench] autotune: best tile {shapes[best]} ms per tile {tile_ms}", file=sys.stderr, flush=True)
+        best, measured, tile_ms = asset.autotune_table(nt, reps=2, times=True)
+        tile_source = "measured" if measured else "table"
+        tile_ms = tile_ms or [None] * len(shapes)
+    tuning_key = asset.tuning_key(nt)
+    print(f"[bench] tile {shapes[best]} ({tile_source}, key '{tuning_key}') ms per tile {tile_ms}", file=sys.stderr,
+          flush=True)
     out = hip.DeviceArray((nt, info.controlsPerFrame), np.float32, s)
     for _ in range(args.warmup):
         asset.frame_cache_into(out, 0, nt)
@@ -781,6 +793,7 @@ def main():
                    "controls_per_frame": info.controlsPerFrame, "payload_mb": round(info.payloadBytes / 1e6, 2),
                    "fast_leaves": info.fast_leaves,
                    "tile": {"workgroup_slots": shapes[best][0], "lds_window_bytes": shapes[best][1],
+                            "source": tile_source, "tuning_key": tuning_key,
                             "autotune_ms": dict(zip([f"{b}x{w}" for b, w in shapes], tile_ms))}},
         "hbm_roofline_pct": round(100.0 * achieved / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -847,6 +860,10 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data, F, args.cpu_seconds, HT)
     asset.close()
+    if args.save_tuning:
+        hip.tuning_save(args.save_tuning)
+        print(f"[bench] tuning table: {hip.tuning_count()} entries written to {args.save_tuning}", file=sys.stderr,
+              flush=True)
     print(json.dumps(line), flush=True)
 
 

@@ -276,6 +276,28 @@ int vbt_asset_set_cache_tile(vbt_asset a, int tile);
 int vbt_asset_get_cache_tile(vbt_asset a, int* tile);
 int vbt_asset_autotune(vbt_asset a, int nt, int reps, float* msPerTile, int* best, void* stream);
 
+/* The tuning table: tile-shape picks keyed by the asset statistics the best
+ * shape depends on, one table per process, kept in a JSON file between runs.
+ * The key (vbt_tuning_key, at most 256 bytes) is "p<profile> nt<frames per
+ * launch> s<round(log2 slots per frame)> r<Grid4>.<Fine6>.<Mode2> k<h0>...<h8>":
+ * route mix in tenths of the active leaves, and the keyframe-count histogram
+ * (0, 1, 2-3, 4-7, ..., 128-255 keyframes per slot) in tenths of the slots;
+ * it runs one kernel on `stream` and waits for it.
+ * vbt_asset_autotune_table sets the tile from the table when the key is
+ * there (*measured = 0, msPerTile untouched), else times every shape as
+ * vbt_asset_autotune does (msPerTile as there) and records the pick
+ * (*measured = 1).  vbt_tuning_load merges a file's
+ * entries into the table; vbt_tuning_save writes the table (atomically, via a
+ * temporary file).  Scientific assets have one kernel shape and no entries.
+ * No counterpart in the reference. */
+int vbt_tuning_key(vbt_asset a, int nt, char* buf, size_t len, void* stream);
+int vbt_asset_autotune_table(vbt_asset a, int nt, int reps, float* msPerTile, int* best, int* measured,
+                             void* stream);
+int vbt_tuning_load(const char* path);
+int vbt_tuning_save(const char* path);
+int vbt_tuning_count(void);
+int vbt_tuning_clear(void);
+
 /* ------------------------------------------------ batched frame caches -- */
 /* Frame-cache builds of several assets as one hipGraph replay: the render
  * jobs (any mix of Grid4 / Fine6 / Mode2 leaves) as ONE persistent kernel over

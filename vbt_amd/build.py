@@ -22,7 +22,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["format.hip", "decode.hip", "server.hip", "batch.hip", "march.hip", "cycles.hip"]
+HIP_SOURCES = ["format.hip", "decode.hip", "server.hip", "batch.hip", "march.hip", "cycles.hip", "tuning.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

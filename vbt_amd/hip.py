@@ -178,6 +178,14 @@ def lib():
         L.vbt_asset_get_cache_tile.argtypes = [c_void_p, ctypes.POINTER(c_int)]
         L.vbt_asset_autotune.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(c_int), c_void_p]
+        L.vbt_tuning_key.argtypes = [c_void_p, c_int, ctypes.c_char_p, ctypes.c_size_t, c_void_p]
+        L.vbt_asset_autotune_table.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(ctypes.c_float),
+                                               ctypes.POINTER(c_int), ctypes.POINTER(c_int), c_void_p]
+        L.vbt_tuning_load.argtypes = [ctypes.c_char_p]
+        L.vbt_tuning_save.argtypes = [ctypes.c_char_p]
+        L.vbt_tuning_count.restype = c_int
+        L.vbt_tuning_count.argtypes = []
+        L.vbt_tuning_clear.argtypes = []
         L.vbt_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
         L.vbt_server_default_config.restype = None
         L.vbt_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(c_void_p)]
@@ -435,6 +443,25 @@ class Asset:
         check(lib().vbt_asset_autotune(self.h, int(nt or self.dims[3]), reps, ms, ctypes.byref(best), self.stream.h))
         return best.value, [round(float(x), 4) for x in ms]
 
+    def tuning_key(self, nt: int | None = None) -> str:
+        """This asset's key in the tuning table for nt frames per launch."""
+        buf = ctypes.create_string_buffer(256)
+        check(lib().vbt_tuning_key(self.h, int(nt or self.dims[3]), buf, len(buf), self.stream.h))
+        return buf.value.decode()
+
+    def autotune_table(self, nt: int | None = None, reps: int = 3, times: bool = False):
+        """The tile from the tuning table when this asset's key is in it, else
+        a timed sweep whose pick joins the table; returns (tile, measured), or
+        (tile, measured, [ms per tile] or None) with times=True."""
+        n = lib().vbt_cache_tile_count()
+        ms = (ctypes.c_float * n)()
+        best, measured = c_int(), c_int()
+        check(lib().vbt_asset_autotune_table(self.h, int(nt or self.dims[3]), reps, ms, ctypes.byref(best),
+                                             ctypes.byref(measured), self.stream.h))
+        if not times:
+            return best.value, bool(measured.value)
+        return best.value, bool(measured.value), [round(float(x), 4) for x in ms] if measured.value else None
+
     def frame_cache_into(self, out: DeviceArray, t0: int, nt: int, layout=CACHE_PACKED, stream: Stream | None = None):
         check(lib().vbt_decode_frame_cache(self.h, t0, nt, layout, out.ptr, (stream or self.stream).h))
 
@@ -520,6 +547,23 @@ class Asset:
         out = DeviceArray((n, 8), np.float32, self.stream)
         self.march_into(cache576, params, d, n, out)
         return out.host()
+
+
+def tuning_load(path: str):
+    """Merge a tuning table file into the process's table."""
+    check(lib().vbt_tuning_load(os.fsencode(path)))
+
+
+def tuning_save(path: str):
+    check(lib().vbt_tuning_save(os.fsencode(path)))
+
+
+def tuning_count() -> int:
+    return lib().vbt_tuning_count()
+
+
+def tuning_clear():
+    check(lib().vbt_tuning_clear())
 
 
 def status_message(code: int) -> str:
