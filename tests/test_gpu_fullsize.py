@@ -62,7 +62,7 @@ def test_liquid320_levelset_sign_and_parity():
         assert not (flips & (np.abs(src[sb]) > 0.25) & (np.abs(g[sb]) > 0.01)).any(), f"t={t}"
     # packed cache: a run of frames in one launch
     cpf = a.info.controlsPerFrame
-    out = hip.DeviceArray((8, cpf), np.float32)
+    out = hip.DeviceArray((8, cpf), np.float32, a.stream)  # read back after the asset's stream
     a.frame_cache_into(out, 71, 8)
     for i in (0, 4, 7):
         bitexact(out.host_slice(i * cpf, cpf), o.cache_packed(71 + i)[0], f"liquid320 cache t={71 + i}")

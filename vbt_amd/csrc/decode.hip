@@ -373,13 +373,42 @@ __global__ void k_group_leaf(const uint64_t* ctrl, uint32_t A, uint64_t total, u
     groupLeaf[b] = g < total ? find_leaf(ctrl, A, g) : (A ? A - 1u : 0u);
 }
 
+// The slot plan and group ranges of a render asset's packed cache (see
+// slot_plan): one thread per slot, blocks of 256 = 4 whole groups.
+__global__ void __launch_bounds__(256) k_slot_plan(AssetDev a, uint64_t total, uint2* plan, uint2* groupRange)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    if (g < total) {
+        const uint2 p = slot_plan(a, g);
+        plan[g] = p;
+        const uint32_t kind = plan_kind(p), keyCount = p.y & 0xFFu;
+        if (plan_swept(kind)) {
+            lo = p.x;
+            hi = p.x + keyCount * (kind == PLAN_FP32 ? 8u : 3u);
+        } else if (kind == PLAN_SLOW) {
+            lo = 0u;
+            hi = 0xFFFFFFFFu;
+        }
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if ((threadIdx.x & 63u) == 0u && g < total) groupRange[g / CACHE_GROUP] = make_uint2(lo, hi);
+}
+
 template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
-__global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32_t* groupLeaf, int t0, int nt,
+__global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32_t* tileGroup, int t0, int nt,
                                                         float* out, uint64_t total, uint32_t* errFlag)
 {
     __shared__ uint32_t lds[WINDOW / 4 + 4]; // + 4: reads past the window, see sweep_frames
     __shared__ uint32_t sRed[BLOCK / 64];
-    cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed);
+    if (LAYOUT == VBT_CACHE_PACKED) {
+        // tile = groups [tileGroup[i], tileGroup[i + 1]) (plan_tiles)
+        const uint32_t g0 = tileGroup[blockIdx.x], g1 = tileGroup[blockIdx.x + 1u];
+        cache_plan_tile<BLOCK, WINDOW>(a, g0, g1 - g0, t0, nt, out, total, errFlag, lds, sRed);
+    } else {
+        cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, a.groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed);
+    }
 }
 
 // Few frames: one thread per cache slot, the reference bracket search per
@@ -597,14 +626,17 @@ __global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_
     }
 }
 
+// Packed layout: grid = the shape's tile list (tileGroup, nTiles entries + 1);
+// 576 layout: one tile per BLOCK slots.
 template <int LAYOUT>
-void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* groupLeaf, int t0, int nt, float* out,
-                         uint64_t total, uint32_t* err, hipStream_t s)
+void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* tileGroup, uint32_t nTiles, int t0, int nt,
+                         float* out, uint64_t total, uint32_t* err, hipStream_t s)
 {
     switch (tile) {
 #define VBT_TILE(i, B, W)                                                                                   \
     case i:                                                                                                 \
-        k_cache_render<LAYOUT, B, W><<<blocks_for(total, B), B, 0, s>>>(ad, groupLeaf, t0, nt, out, total, err); \
+        k_cache_render<LAYOUT, B, W><<<LAYOUT == VBT_CACHE_PACKED ? nTiles : blocks_for(total, B), B, 0, s>>>( \
+            ad, tileGroup, t0, nt, out, total, err);                                                        \
         break;
         VBT_TILE(0, 64, 8192)
         VBT_TILE(1, 64, 16384)
@@ -1145,6 +1177,32 @@ __global__ void k_fill(float* p, uint64_t n, float v)
 
 } // namespace
 
+// Cuts the packed cache into tiles of at most maxGroups CACHE_GROUP-slot
+// groups whose swept records span at most `window` bytes (greedy, in slot
+// order; a group that alone exceeds the window is a tile of its own and takes
+// the multi-pass staging).  Appends the tiles' first groups and one past the
+// last group to `out`; returns the tile count.
+static uint32_t plan_tiles(const std::vector<uint2>& gr, uint32_t maxGroups, uint32_t window, std::vector<uint32_t>& out)
+{
+    const uint32_t ng = (uint32_t)gr.size();
+    uint32_t tiles = 0, i = 0;
+    while (i < ng) {
+        out.push_back(i);
+        ++tiles;
+        uint32_t lo = gr[i].x, hi = gr[i].y, j = i + 1;
+        while (j < ng && j - i < maxGroups) {
+            const uint32_t nlo = std::min(lo, gr[j].x), nhi = std::max(hi, gr[j].y);
+            if (nlo != 0xFFFFFFFFu && nhi - (nlo & ~3u) > window) break;
+            lo = nlo;
+            hi = nhi;
+            ++j;
+        }
+        i = j;
+    }
+    out.push_back(ng);
+    return tiles;
+}
+
 // ============================================================== assets ==
 struct vbt_asset_s {
     vbt_file_header h{};
@@ -1159,6 +1217,12 @@ struct vbt_asset_s {
     float* d_basis = nullptr;
     uint32_t* d_err = nullptr;   // error flag word for launches
     uint32_t* d_blockLeaf = nullptr; // first active leaf of each 64-slot group of the packed cache
+    uint2* d_slotPlan = nullptr;     // render: slot plan of the packed cache (k_slot_plan)
+    uint2* d_groupRange = nullptr;   // render: swept-record byte range of each 64-slot group
+    // render: the packed cache cut into tiles per tile shape (plan_tiles):
+    // shape i's tile j is groups [d_tiles[tileOff[i] + j], d_tiles[tileOff[i] + j + 1])
+    uint32_t* d_tiles = nullptr;
+    uint32_t tileOff[CACHE_TILE_COUNT] = {}, tileCount[CACHE_TILE_COUNT] = {};
     int cacheTile = CACHE_TILE_DEFAULT;
     Pair* d_blocks = nullptr;
     VbtFrameIndex index; // the asset's own frame index (vbt_asset_build_frame_index)
@@ -1191,6 +1255,8 @@ struct vbt_asset_s {
         a.scaleCount = h.coarseAcScaleCount;
         a.payloadBytes = payloadBytes;
         a.groupLeaf = d_blockLeaf;
+        a.slotPlan = d_slotPlan;
+        a.groupRange = d_groupRange;
         a.controlsPerFrame = controlsPerFrame;
         return a;
     }
@@ -1211,7 +1277,8 @@ void free_asset(vbt_asset a)
     if (!a) return;
     vbt_asset_drop_frame_index(a);
     void* ptrs[] = {a->d_offsets, a->d_payload, a->d_scales, a->d_compact, a->d_status,
-                    a->d_leaves,  a->d_ctrl,    a->d_basis,  a->d_err,     a->d_blocks, a->d_blockLeaf};
+                    a->d_leaves,  a->d_ctrl,    a->d_basis,  a->d_err,     a->d_blocks, a->d_blockLeaf,
+                    a->d_slotPlan, a->d_groupRange, a->d_tiles};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete a;
@@ -1458,6 +1525,26 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
         k_group_leaf<<<blocks_for(ng + 1ull, 256), 256, 0, s>>>(a->d_ctrl, a->A, a->controlsPerFrame, ng,
                                                                  a->d_blockLeaf);
         TRY(hipGetLastError());
+        if (h.profileType != 2 && a->controlsPerFrame) {
+            if ((rc = dalloc(a, &a->d_slotPlan, (size_t)a->controlsPerFrame)) ||
+                (rc = dalloc(a, &a->d_groupRange, (size_t)ng)))
+                return fail(rc);
+            ad = a->dev();
+            k_slot_plan<<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(ad, a->controlsPerFrame, a->d_slotPlan,
+                                                                            a->d_groupRange);
+            TRY(hipGetLastError());
+            std::vector<uint2> gr(ng);
+            TRY(hipMemcpyAsync(gr.data(), a->d_groupRange, 8ull * ng, hipMemcpyDeviceToHost, s));
+            TRY(hipStreamSynchronize(s));
+            std::vector<uint32_t> all;
+            for (int i = 0; i < CACHE_TILE_COUNT; ++i) {
+                a->tileOff[i] = (uint32_t)all.size();
+                a->tileCount[i] = plan_tiles(gr, CACHE_TILES[i].block / CACHE_GROUP, CACHE_TILES[i].window, all);
+            }
+            if ((rc = dalloc(a, &a->d_tiles, all.size()))) return fail(rc);
+            TRY(hipMemcpyAsync(a->d_tiles, all.data(), 4ull * all.size(), hipMemcpyHostToDevice, s));
+            TRY(hipStreamSynchronize(s));
+        }
     }
     // host-side statistics of the page table
     std::vector<uint32_t> st(L);
@@ -1556,8 +1643,9 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
             k_cache_exact<VBT_CACHE_PACKED><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
                 ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
         else
-            launch_cache_render<VBT_CACHE_PACKED>(a->cacheTile, ad, a->d_blockLeaf, t0, nt, d_out,
-                                                  a->controlsPerFrame, a->d_err, s);
+            launch_cache_render<VBT_CACHE_PACKED>(a->cacheTile, ad, a->d_tiles + a->tileOff[a->cacheTile],
+                                                  a->tileCount[a->cacheTile], t0, nt, d_out, a->controlsPerFrame,
+                                                  a->d_err, s);
     } else {
         const uint64_t n = (uint64_t)a->A * 576u * (uint64_t)nt;
         k_fill<<<(uint32_t)std::min<uint64_t>(blocks_for(n, 256), 65535), 256, 0, s>>>(d_out, n, 0.0f);
@@ -1565,7 +1653,7 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
             k_cache_exact<VBT_CACHE_STRIDE576><<<blocks_for(a->controlsPerFrame, 256), 256, 0, s>>>(
                 ad, t0, nt, d_out, a->controlsPerFrame, a->d_err);
         else
-            launch_cache_render<VBT_CACHE_STRIDE576>(a->cacheTile, ad, a->d_blockLeaf, t0, nt, d_out,
+            launch_cache_render<VBT_CACHE_STRIDE576>(a->cacheTile, ad, nullptr, 0u, t0, nt, d_out,
                                                      a->controlsPerFrame, a->d_err, s);
     }
     HIPCHK(hipGetLastError());
@@ -1724,11 +1812,12 @@ int vbt_asset_autotune(vbt_asset a, int nt, int reps, float* msPerTile, int* bes
     int bestTile = a->cacheTile;
     float bestMs = 1e30f;
     for (int tile = 0; tile < CACHE_TILE_COUNT; ++tile) {
-        launch_cache_render<VBT_CACHE_PACKED>(tile, ad, a->d_blockLeaf, 0, nt, scratch, a->controlsPerFrame, a->d_err, s);
+        launch_cache_render<VBT_CACHE_PACKED>(tile, ad, a->d_tiles + a->tileOff[tile], a->tileCount[tile], 0, nt,
+                                              scratch, a->controlsPerFrame, a->d_err, s);
         HIPCHK(hipEventRecord(e0, s));
         for (int r = 0; r < reps; ++r)
-            launch_cache_render<VBT_CACHE_PACKED>(tile, ad, a->d_blockLeaf, 0, nt, scratch, a->controlsPerFrame,
-                                                  a->d_err, s);
+            launch_cache_render<VBT_CACHE_PACKED>(tile, ad, a->d_tiles + a->tileOff[tile], a->tileCount[tile], 0,
+                                                  nt, scratch, a->controlsPerFrame, a->d_err, s);
         HIPCHK(hipEventRecord(e1, s));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.0f;

@@ -52,6 +52,8 @@ struct AssetDev {
     const uint64_t* ctrl;
     const float* basis;
     const uint32_t* groupLeaf;  // first active leaf of each CACHE_GROUP-slot group
+    const uint2* slotPlan;      // render: per packed-cache slot, where and how its records are swept
+    const uint2* groupRange;    // render: per CACHE_GROUP-slot group, byte range of its swept records
     uint32_t W, H, D, F, L, A, profile, scaleCount;
     uint64_t payloadBytes;
     uint64_t controlsPerFrame;
@@ -592,7 +594,9 @@ __device__ __forceinline__ void sweep_start(lds_byte* L, uint32_t off, uint32_t 
     s.y = sweep_recip(s.gap);
 }
 
-// Cache-policy bits of a buffer store on gfx950: NT (non-temporal).
+// Cache-policy bits of a buffer store on gfx950: 0 (write-back through L2)
+// or NT (non-temporal: streamed, no cache allocation).
+constexpr int STORE_WB = 0;
 constexpr int STORE_NT = 2;
 
 // Raw buffer descriptor of one frame of the output (base, size in bytes):
@@ -655,62 +659,109 @@ __device__ __forceinline__ void key_decode_f(const RawKey& r, uint32_t off, floa
 // KB + 7 bytes past the window's staged records: record k + 2 <= last + 2
 // starts at most KB bytes past the control's end, which is inside the
 // window, and the read covers two dwords from there.
+// The loop state of one control's sweep (the same fields for every record
+// format, so lanes of different formats can share one loop around it) and its
+// steps: sweep_begin sets it up at the first frame, sweep_update moves it to
+// the current frame, sweep_value produces that frame's control.
+struct SweepRegs {
+    float fk, fn, vk, vn, gap, y;
+    float fkS;   // the output's key frame: past every frame at the last key
+    float frame; // the current frame
+    uint32_t pP, e2;
+    bool more;   // key k + 1 exists
+    RawKey raw;  // record k + 2, read a step ahead
+};
+
+template <uint32_t KB>
+__device__ __forceinline__ void sweep_begin(lds_byte* L, uint32_t off, uint32_t last, const SweepState& s, int tBegin,
+                                            SweepRegs& r)
+{
+    r.fk = (float)s.fk;
+    r.fn = (float)s.fn;
+    r.vk = s.vk;
+    r.vn = s.vn;
+    r.gap = s.gap;
+    r.y = s.y;
+    r.pP = off + (s.k + 2u) * KB;
+    r.e2 = off + (last + 2u) * KB;
+    r.more = s.k < last;
+    r.raw = lds_fetch(L, r.pP);
+    // one comparison fkS >= frame selects vk both before the first key and
+    // after the last
+    r.fkS = r.more ? r.fk : __builtin_inff();
+    r.frame = (float)tBegin;
+}
+
 template <uint32_t KB, bool NANFIX>
+__device__ __forceinline__ void sweep_update(lds_byte* L, SweepRegs& r)
+{
+    float fp, vp;
+    // the record read a step ago is consumed here on every lane: left to
+    // itself the compiler re-reads it on advance only, behind a branch that
+    // waits for the read
+    asm volatile("" ::"v"(r.raw.lo), "v"(r.raw.hi));
+    key_decode_f<KB, NANFIX>(r.raw, r.pP, fp, vp);
+    asm volatile("" : "+v"(fp), "+v"(vp));
+    const bool adv = r.more & (r.fn <= r.frame);
+    const float gNew = fp - r.fn;
+    const float yNew = sweep_recip(gNew);
+    r.fk = adv ? r.fn : r.fk;
+    r.vk = adv ? r.vn : r.vk;
+    r.fn = adv ? fp : r.fn;
+    r.vn = adv ? vp : r.vn;
+    r.gap = adv ? gNew : r.gap;
+    r.y = adv ? yNew : r.y;
+    r.pP += adv ? KB : 0u;
+    r.raw = lds_fetch(L, r.pP);
+    r.more = r.pP < r.e2;
+    r.fkS = r.more ? r.fk : __builtin_inff();
+}
+
+__device__ __forceinline__ float sweep_value(const SweepRegs& r)
+{
+    const float alpha = sweep_alpha(r.frame - r.fkS, r.gap, r.y); // frame - fkS: exact (integers < 2^16)
+    const float lerp = r.vk + alpha * (r.vn - r.vk);
+    return r.fkS >= r.frame ? r.vk : lerp;
+}
+
+// Frames tBegin .. tBegin+cn-1 (all inside [0, F)) of one control from state
+// s at frame tBegin; its records are in LDS at byte offset off (record i at
+// off + i * KB).  Frame i goes to out + i * strideBytes + dstBytes, with out
+// the same for every lane: a buffer store (cache policy POLICY) with the
+// frame's base in scalar registers and the lane's 32-bit offset (strideBytes
+// < 2^32, dstBytes < strideBytes; the host checks the frame size).
+// The state update is branch-free.  Key frames are carried as floats (exact
+// integers; the frame counter is a wave-uniform float), so the weight's
+// numerator and the key gap are single float subtractions.  The key counter
+// is replaced by the byte offset pP of record k + 2 (the record read ahead):
+// k < last exactly when pP < e2 = off + (last + 2) * KB.  Record k + 2 is
+// read every step and selected on advance.  Step i issues that LDS read
+// first and produces frame i - 1's output while the read is in flight, then
+// updates the state for frame i.  Once k + 2 passes the last record the read
+// returns whatever follows it in LDS (the next control's records or
+// padding); such a record only becomes key k + 1 when k reaches the last
+// key, where the output is vk and never uses it.  The read reaches at most
+// KB + 7 bytes past the window's staged records: record k + 2 <= last + 2
+// starts at most KB bytes past the control's end, which is inside the
+// window, and the read covers two dwords from there.
+template <uint32_t KB, bool NANFIX, int POLICY = STORE_NT>
 __device__ __forceinline__ void sweep_frames(lds_byte* L, uint32_t off, uint32_t last, const SweepState& s,
                                              int tBegin, int cn, char* out, uint32_t dstBytes, uint64_t strideBytes)
 {
-    const float INF = __builtin_inff();
-    float fk = (float)s.fk, fn = (float)s.fn;
-    float vk = s.vk, vn = s.vn;
-    float gap = s.gap, y = s.y;
-    uint32_t pP = off + (s.k + 2u) * KB;
-    const uint32_t e2 = off + (last + 2u) * KB;
-    bool more = s.k < last; // key k + 1 exists
-    RawKey raw = lds_fetch(L, pP);
-    // the output's key frame: past every frame at the last key, so the one
-    // comparison fkS >= frame selects vk both before the first key and after
-    // the last
-    float fkS = more ? fk : INF;
-    float frame = (float)tBegin;
-    auto update = [&]() {
-        float fp, vp;
-        // the record read a step ago is consumed here on every lane: left to
-        // itself the compiler re-reads it on advance only, behind a branch
-        // that waits for the read
-        asm volatile("" ::"v"(raw.lo), "v"(raw.hi));
-        key_decode_f<KB, NANFIX>(raw, pP, fp, vp);
-        asm volatile("" : "+v"(fp), "+v"(vp));
-        const bool adv = more & (fn <= frame);
-        const float gNew = fp - fn;
-        const float yNew = sweep_recip(gNew);
-        fk = adv ? fn : fk;
-        vk = adv ? vn : vk;
-        fn = adv ? fp : fn;
-        vn = adv ? vp : vn;
-        gap = adv ? gNew : gap;
-        y = adv ? yNew : y;
-        pP += adv ? KB : 0u;
-        raw = lds_fetch(L, pP);
-        more = pP < e2;
-        fkS = more ? fk : INF;
-    };
+    SweepRegs r;
+    sweep_begin<KB>(L, off, last, s, tBegin, r);
     auto output = [&]() {
-        const float alpha = sweep_alpha(frame - fkS, gap, y); // frame - fkS: exact (integers < 2^16)
-        const float lerp = vk + alpha * (vn - vk);
-        const float val = fkS >= frame ? vk : lerp;
-        // non-temporal store (the cache is written once here and read by
-        // later launches): 8 % faster on smoke256 than a default store
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(val), frame_rsrc(out, (uint32_t)strideBytes),
-                                              (int)dstBytes, 0, STORE_NT);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(sweep_value(r)), frame_rsrc(out, (uint32_t)strideBytes),
+                                              (int)dstBytes, 0, POLICY);
         out += strideBytes;
     };
     // one basic block per step: the record read at the end of a step stays
     // there, and the next step's output stage runs while it is in flight
-    update();
+    sweep_update<KB, NANFIX>(L, r);
     for (int i = 1; i < cn; ++i) {
         output();
-        frame += 1.0f;
-        update();
+        r.frame += 1.0f;
+        sweep_update<KB, NANFIX>(L, r);
     }
     output();
 }
@@ -888,6 +939,174 @@ __device__ __forceinline__ void cache_render_tile(const AssetDev& a, const uint3
             for (uint32_t j = 0; j < SU; ++j)
                 if (i0 + j * BLOCK < nw) lds[i0 + j * BLOCK] = v[j];
         }
+        __syncthreads();
+        if (mine) {
+            run(kLo - w0 * 4u);
+            pending = false;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------- slot plan of the packed cache --
+// Per packed-cache slot, computed once per render asset (k_slot_plan): where
+// the slot's keyframe records start and how the sweep reads them, so a tile's
+// lanes start staging after one load instead of a walk over the control
+// offsets, the leaf's plan and its descriptor.
+//   x: byte offset in the payload of the slot's first keyframe record (swept
+//      slots), or the slot's active leaf index (other slots)
+//   y: keyCount | kind << 8
+// PLAN_ZERO is a stream without keys (the control is 0 at every frame);
+// PLAN_SLOW covers poisoned leaves and keys that do not strictly increase:
+// those slots take the per-frame path of cache_slot_frames_slow.
+// The group range of every CACHE_GROUP consecutive slots is the byte range
+// [x, x + keyCount * record bytes) spanned by its swept slots (x = ~0 when it
+// has none); a group holding a PLAN_SLOW slot gets the range [0, ~0), which
+// no window holds, so its tiles take the general path.  A tile reads its
+// groups' ranges to size its LDS window.
+constexpr uint32_t PLAN_SLOW = 0u, PLAN_FP16 = 1u, PLAN_FP16_NAN = 2u, PLAN_FP32 = 3u, PLAN_ZERO = 4u;
+__host__ __device__ inline bool plan_swept(uint32_t kind) { return kind >= PLAN_FP16 && kind <= PLAN_FP32; }
+__host__ __device__ inline uint32_t plan_kind(uint2 p) { return (p.y >> 8) & 7u; }
+
+__device__ inline uint2 slot_plan(const AssetDev& a, uint64_t g)
+{
+    uint32_t c = a.groupLeaf[g / CACHE_GROUP];
+    while (c + 1u < a.A && a.ctrl[c + 1u] <= g) ++c;
+    const DevLeaf& d = a.leaves[c];
+    const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
+    const uint32_t status = d.status;
+    const bool isCoarse = slot < d.cc;
+    const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
+    const uint32_t ci = isCoarse ? slot : slot - d.cc;
+    if ((status & STATUS_MASK) != LEAF_OK) return make_uint2(c, PLAN_SLOW << 8);
+    const uint32_t desc = *(const uint32_t*)(a.payload + d.word * 4u + s.base + ci * 4u); // 4-aligned
+    const uint32_t keyStart = desc & 0xFFFFu, keyCount = (desc >> 16) & 0xFFu;
+    const uint32_t kb = keyframe_bytes(s.codec);
+    if (keyCount == 0u) return make_uint2(c, PLAN_ZERO << 8);
+    if (!(status & FLAG_SWEEP)) return make_uint2(c, keyCount | PLAN_SLOW << 8);
+    const uint32_t kind = kb == 8u ? PLAN_FP32 : ((status & FLAG_NO_NAN) ? PLAN_FP16 : PLAN_FP16_NAN);
+    return make_uint2(d.word * 4u + s.keyBase + keyStart * kb, keyCount | kind << 8);
+}
+
+// Per-frame path of a slot the sweep does not take (poisoned leaf: NaN and
+// the error flag; no keys: 0; keys not strictly increasing: the reference's
+// bracket search per frame from global memory).
+__device__ __forceinline__ void cache_slot_frames_slow(const AssetDev& a, uint32_t c, uint64_t g, int t0, int nt,
+                                                       float* o, uint64_t stride, uint32_t* errFlag)
+{
+    const DevLeaf& d = a.leaves[c];
+    const uint32_t slot = (uint32_t)(g - a.ctrl[c]);
+    if ((d.status & STATUS_MASK) != LEAF_OK) {
+        for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = __builtin_nanf("");
+        atomicOr(errFlag, 1u);
+        return;
+    }
+    const bool isCoarse = slot < d.cc;
+    const StreamRef s = isCoarse ? d.plan.coarse : (d.plan.mode == MODE_FINE6 ? d.plan.fine : d.plan.shell);
+    const uint32_t ci = isCoarse ? slot : slot - d.cc;
+    const Bytes b{a.payload + d.word * 4u};
+    const uint32_t desc = b.u32(s.base + ci * 4u);
+    const uint32_t keyStart = desc & 0xFFFFu, keyCount = (desc >> 16) & 0xFFu;
+    if (keyCount == 0u) {
+        for (int i = 0; i < nt; ++i) o[(uint64_t)i * stride] = 0.0f;
+        return;
+    }
+    const uint32_t binBase = s.base + s.count * 4u + ci * s.bins;
+    for (int i = 0; i < nt; ++i)
+        o[(uint64_t)i * stride] = decode_sequence(b, s.keyBase, s.codec, keyStart, keyCount, binBase, s.bins,
+                                                  (int)a.F, t0 + i);
+}
+
+// Copy payload dwords [w0, w0 + nw) into lds[0 .. nw) with STAGE loads in
+// flight per lane.
+template <uint32_t BLOCK, uint32_t STAGE>
+__device__ __forceinline__ void stage_window(const uint8_t* payload, uint32_t w0, uint32_t nw, uint32_t* lds,
+                                             uint32_t tid)
+{
+    const uint32_t* pw = (const uint32_t*)payload + w0;
+    for (uint32_t i0 = tid; i0 < nw; i0 += STAGE * BLOCK) {
+        uint32_t v[STAGE];
+#pragma unroll
+        for (uint32_t j = 0; j < STAGE; ++j) v[j] = i0 + j * BLOCK < nw ? pw[i0 + j * BLOCK] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < STAGE; ++j)
+            if (i0 + j * BLOCK < nw) lds[i0 + j * BLOCK] = v[j];
+    }
+}
+
+// One tile of the packed cache from the slot plan: the same results as
+// cache_render_tile (VBT_CACHE_PACKED).  The tile is the CACHE_GROUP-slot
+// groups [g0, g0 + ng) (ng <= BLOCK / CACHE_GROUP; the host cuts the packed
+// cache into tiles whose records fit the window where a group allows it, see
+// plan_tiles).  Its record range comes from its groups' ranges: a tile that
+// fits stages its records in one pass with a single barrier, others take the
+// multi-pass staging of cache_render_tile.  Each lane stores its control's
+// frames (write-back stores: the tile's row segments share lines with their
+// neighbours' and merge in L2).
+template <uint32_t BLOCK, uint32_t WINDOW, uint32_t STAGE = 32u>
+__device__ __forceinline__ void cache_plan_tile(const AssetDev& a, uint32_t g0, uint32_t ng, int t0, int nt,
+                                                float* out, uint64_t total, uint32_t* errFlag, uint32_t* lds,
+                                                uint32_t* sRed)
+{
+    constexpr uint32_t NW = BLOCK / 64;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint64_t g = (uint64_t)g0 * CACHE_GROUP + tid;
+    const bool live = tid < ng * CACHE_GROUP && g < total;
+    uint32_t tlo = 0xFFFFFFFFu, thi = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        if (w < ng) {
+            const uint2 r = a.groupRange[g0 + w];
+            tlo = min(tlo, r.x);
+            thi = max(thi, r.y);
+        }
+    }
+    const uint2 p = live ? a.slotPlan[g] : make_uint2(0u, PLAN_ZERO << 8);
+    const uint32_t kind = plan_kind(p), keyCount = p.y & 0xFFu;
+    const bool sw = live && plan_swept(kind);
+    const uint32_t kb = kind == PLAN_FP32 ? 8u : 3u;
+    const uint32_t kLo = p.x, kHi = p.x + keyCount * kb;
+    const uint32_t last = keyCount - 1u;
+    char* const ob = (char*)out;
+    const uint64_t strideBytes = total * 4u;
+    const uint32_t dstBytes = (uint32_t)g * 4u;
+    lds_byte* L = (lds_byte*)lds;
+    if (live && !sw) cache_slot_frames_slow(a, p.x, g, t0, nt, out + g, total, errFlag);
+    auto run = [&](uint32_t off) {
+        SweepState st;
+        if (kind == PLAN_FP32) {
+            sweep_start<8u, false>(L, off, last, t0, st);
+            sweep_frames<8u, false, STORE_WB>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
+        } else if (kind == PLAN_FP16) {
+            sweep_start<3u, false>(L, off, last, t0, st);
+            sweep_frames<3u, false, STORE_WB>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
+        } else {
+            sweep_start<3u, true>(L, off, last, t0, st);
+            sweep_frames<3u, true, STORE_WB>(L, off, last, st, t0, nt, ob, dstBytes, strideBytes);
+        }
+    };
+    if (tlo == 0xFFFFFFFFu) return; // no swept slot in the tile
+    if (thi - (tlo & ~3u) <= WINDOW) {
+        const uint32_t w0 = tlo / 4u;
+        stage_window<BLOCK, STAGE>(a.payload, w0, (thi + 3u) / 4u - w0, lds, tid);
+        __syncthreads();
+        if (sw) run(kLo - w0 * 4u);
+        return;
+    }
+    // multi-pass: as cache_render_tile
+    const uint32_t waveLo = wave_min(sw ? kLo : 0xFFFFFFFFu);
+    const uint32_t waveHi = wave_max(sw ? kHi : 0u);
+    const bool waveFits = waveLo != 0xFFFFFFFFu && waveHi - (waveLo & ~3u) <= WINDOW;
+    bool pending = sw;
+    for (;;) {
+        const uint32_t lo = block_min<NW>(pending ? kLo : 0xFFFFFFFFu, sRed, lane, wave);
+        if (lo == 0xFFFFFFFFu) break;
+        const uint32_t w0 = lo / 4u;
+        const uint32_t limit = w0 * 4u + WINDOW;
+        const bool mine = pending && ((waveFits ? waveHi <= limit : kHi <= limit) || kLo == lo);
+        const uint32_t hi = block_max<NW>(mine ? kHi : 0u, sRed, lane, wave);
+        const uint32_t nw = hi > w0 * 4u ? (hi + 3u) / 4u - w0 : 0u;
+        stage_window<BLOCK, STAGE>(a.payload, w0, nw, lds, tid);
         __syncthreads();
         if (mine) {
             run(kLo - w0 * 4u);

@@ -276,8 +276,11 @@ class DeviceArray:
         arr = np.ascontiguousarray(arr)
         d = cls(arr.shape, arr.dtype, stream)
         check(lib().vbt_memcpy_h2d(d.ptr, arr.ctypes.data, arr.nbytes, stream.h if stream else None))
+        # complete before any stream reads it (assets run on non-blocking streams)
         if stream:
             stream.sync()
+        else:
+            check(lib().vbt_stream_synchronize(None))
         return d
 
     def host(self) -> np.ndarray:
@@ -299,7 +302,11 @@ class DeviceArray:
         return out
 
     def fill_bytes(self, value: int):
+        """Set every byte; an array without a stream is filled before this returns
+        (work on non-blocking streams is not ordered after the null stream)."""
         check(lib().vbt_memset_d(self.ptr, value, self.nbytes, self.stream.h if self.stream else None))
+        if not self.stream:
+            check(lib().vbt_stream_synchronize(None))
 
     def free(self):
         if self.ptr:
