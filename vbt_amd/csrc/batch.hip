@@ -61,7 +61,8 @@ struct BatchJob {
     AssetDev a;
     float* out;
     uint32_t* err;
-    int t0, nt, per; // per: frames per range
+    const uint32_t* tiles; // packed jobs: the asset's tile list of the batch's tile shape
+    int t0, nt, per;       // per: frames per range
     uint32_t kind;
 };
 
@@ -109,13 +110,13 @@ __global__ void __launch_bounds__(256) k_group_cost(AssetDev a, uint32_t* cost)
     if ((threadIdx.x & 63u) == 0u && g < a.controlsPerFrame) cost[g / CACHE_GROUP] = bytes;
 }
 
-// Few frames: the reference bracket search per slot and frame (no staging).
-template <uint32_t BLOCK>
-__device__ __forceinline__ void exact_tile(const AssetDev& a, uint32_t tile, int f0, int n, float* out,
+// Few frames: the reference bracket search per slot and frame (no staging);
+// the tile is the groups [g0, g0 + ng).
+__device__ __forceinline__ void exact_tile(const AssetDev& a, uint32_t g0, uint32_t ng, int f0, int n, float* out,
                                            uint64_t total, uint32_t* err)
 {
-    const uint64_t g = (uint64_t)tile * BLOCK + threadIdx.x;
-    if (g >= total) return;
+    const uint64_t g = (uint64_t)g0 * CACHE_GROUP + threadIdx.x;
+    if (threadIdx.x >= ng * CACHE_GROUP || g >= total) return;
     bool bad = false;
     for (int i = 0; i < n; ++i) {
         int32_t st;
@@ -142,10 +143,9 @@ __device__ __forceinline__ void batch_item(const BatchArgs& A, uint32_t it, uint
     const uint64_t total = a.controlsPerFrame;
     if (J.kind == JOB_PACKED) {
         float* out = J.out + (uint64_t)(f0 - J.t0) * total;
-        if (n <= CACHE_EXACT_MAX_FRAMES) exact_tile<BLOCK>(a, e.y, f0, n, out, total, J.err);
-        else
-            cache_render_tile<VBT_CACHE_PACKED, BLOCK, WINDOW, BATCH_STAGE>(a, a.groupLeaf, e.y, f0, n, out, total, J.err, lds,
-                                                               sRed);
+        const uint32_t g0 = J.tiles[e.y], ng = J.tiles[e.y + 1u] - g0;
+        if (n <= CACHE_EXACT_MAX_FRAMES) exact_tile(a, g0, ng, f0, n, out, total, J.err);
+        else cache_plan_tile<BLOCK, WINDOW, BATCH_STAGE>(a, g0, ng, f0, n, out, total, J.err, lds, sRed);
     } else {
         float* out = J.out + (uint64_t)(f0 - J.t0) * a.A * 576u;
         cache_render_tile<VBT_CACHE_STRIDE576, BLOCK, WINDOW, BATCH_STAGE>(a, a.groupLeaf, e.y, f0, n, out, total, J.err, lds,
@@ -344,15 +344,22 @@ int plan(vbt_cache_batch b, const vbt_batch_shape& shape)
         J.nt = job.nt;
         J.per = (job.nt + (int)R - 1) / (int)R;
         J.kind = b->kinds[j];
+        J.tiles = nullptr;
         if (a.A == 0 || J.kind == JOB_SCI) continue;
         {
-            const uint64_t tiles = (a.controlsPerFrame + BLOCK - 1u) / BLOCK;
+            // packed jobs: the asset's tiles of this shape; 576 jobs: BLOCK slots per tile
+            uint32_t nPlan = 0;
+            const uint32_t* hPlan = nullptr;
+            if (J.kind == JOB_PACKED) J.tiles = vbt_internal_asset_tiles(job.asset, shape.tile, &nPlan, &hPlan);
+            const uint64_t tiles = J.tiles ? nPlan : (a.controlsPerFrame + BLOCK - 1u) / BLOCK;
             if (tiles > 0xffffffffull) return vbt_fail(VBT_ERR_INVALID, "batch: too many tiles");
             const std::vector<uint32_t>& gc = b->groupCost[j];
             for (uint64_t t = 0; t < tiles; ++t) {
+                const uint64_t ga = J.tiles ? hPlan[t] : t * NW;
+                const uint64_t gb = std::min<uint64_t>(J.tiles ? hPlan[t + 1] : (t + 1) * NW, gc.size());
                 uint64_t c = 0; // keyframe records + descriptors + bins: about the tile's byte span
-                for (uint64_t g = t * NW; g < std::min<uint64_t>((t + 1) * NW, gc.size()); ++g) c += gc[g];
-                const Entry e{c + 4ull * BLOCK * (uint64_t)job.nt, (uint32_t)j, (uint32_t)t};
+                for (uint64_t g = ga; g < gb; ++g) c += gc[g];
+                const Entry e{c + 4ull * CACHE_GROUP * (gb - ga) * (uint64_t)job.nt, (uint32_t)j, (uint32_t)t};
                 if (shape.heavyTile >= 0 && c > lightSpan) heavy.push_back(e);
                 else entries.push_back(e);
             }
@@ -922,8 +929,16 @@ int vbt_cache_batch_run_noalloc(const vbt_cache_job* jobs, int nJobs, const vbt_
         J.nt = job.nt;
         J.per = (job.nt + (int)R - 1) / (int)R;
         J.kind = kinds[j];
+        J.tiles = nullptr;
         plan.prefix[j] = (uint32_t)std::min<uint64_t>(tiles, NA_MAX_TILES);
-        if (a.A != 0 && J.kind != JOB_SCI) tiles += (a.controlsPerFrame + BLOCK - 1u) / BLOCK;
+        if (a.A != 0 && J.kind == JOB_PACKED) {
+            uint32_t nPlan = 0;
+            const uint32_t* hPlan = nullptr;
+            J.tiles = vbt_internal_asset_tiles(job.asset, sh.tile, &nPlan, &hPlan);
+            tiles += nPlan;
+        } else if (a.A != 0 && J.kind != JOB_SCI) {
+            tiles += (a.controlsPerFrame + BLOCK - 1u) / BLOCK;
+        }
     }
     if (tiles > NA_MAX_TILES)
         return vbt_fail(VBT_ERR_INVALID, "no-alloc batch: " + std::to_string(tiles) + " leaf tiles of " +

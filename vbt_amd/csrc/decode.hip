@@ -1222,6 +1222,7 @@ struct vbt_asset_s {
     // render: the packed cache cut into tiles per tile shape (plan_tiles):
     // shape i's tile j is groups [d_tiles[tileOff[i] + j], d_tiles[tileOff[i] + j + 1])
     uint32_t* d_tiles = nullptr;
+    std::vector<uint32_t> hTiles; // host copy of d_tiles
     uint32_t tileOff[CACHE_TILE_COUNT] = {}, tileCount[CACHE_TILE_COUNT] = {};
     int cacheTile = CACHE_TILE_DEFAULT;
     Pair* d_blocks = nullptr;
@@ -1430,6 +1431,17 @@ uint64_t vbt_internal_asset_bytes(vbt_asset a) { return a->deviceBytes; }
 int vbt_internal_asset_device(vbt_asset a) { return a->device; }
 uint32_t vbt_internal_asset_bad(vbt_asset a) { return a->bad; }
 uint32_t* vbt_internal_asset_err(vbt_asset a) { return a->d_err; }
+const uint32_t* vbt_internal_asset_tiles(vbt_asset a, int shape, uint32_t* count, const uint32_t** host)
+{
+    if (!a->d_tiles || shape < 0 || shape >= CACHE_TILE_COUNT) {
+        *count = 0;
+        *host = nullptr;
+        return nullptr;
+    }
+    *count = a->tileCount[shape];
+    *host = a->hTiles.data() + a->tileOff[shape];
+    return a->d_tiles + a->tileOff[shape];
+}
 int vbt_internal_report_bad(vbt_asset a, const char* what) { return report_bad(a, what); }
 const vbt_file_header* vbt_internal_asset_header(vbt_asset a) { return &a->h; }
 
@@ -1544,6 +1556,7 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
             if ((rc = dalloc(a, &a->d_tiles, all.size()))) return fail(rc);
             TRY(hipMemcpyAsync(a->d_tiles, all.data(), 4ull * all.size(), hipMemcpyHostToDevice, s));
             TRY(hipStreamSynchronize(s));
+            a->hTiles = std::move(all);
         }
     }
     // host-side statistics of the page table

@@ -20,6 +20,11 @@ uint32_t vbt_internal_asset_bad(vbt_asset a);
 uint32_t* vbt_internal_asset_err(vbt_asset a);
 int vbt_internal_report_bad(vbt_asset a, const char* what);
 const vbt_file_header* vbt_internal_asset_header(vbt_asset a);
+// The packed-cache tile list of tile shape `shape` (render assets): tile j is
+// the CACHE_GROUP-slot groups [tiles[j], tiles[j + 1]); *count tiles, the
+// device list returned, the host copy in *host (count + 1 entries).  Scientific
+// assets have none (count 0, nullptr).
+const uint32_t* vbt_internal_asset_tiles(vbt_asset a, int shape, uint32_t* count, const uint32_t** host);
 
 // A frame index of one asset: per bin of the frame range, each packed-cache
 // slot's run of keyframe records (offsets bins x (slots + 1), then the

@@ -1091,6 +1091,7 @@ __device__ __forceinline__ void cache_plan_tile(const AssetDev& a, uint32_t g0, 
         stage_window<BLOCK, STAGE>(a.payload, w0, (thi + 3u) / 4u - w0, lds, tid);
         __syncthreads();
         if (sw) run(kLo - w0 * 4u);
+        __syncthreads(); // the window is free again (work-queue kernels stage their next tile into it)
         return;
     }
     // multi-pass: as cache_render_tile
