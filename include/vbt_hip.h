@@ -176,6 +176,9 @@ enum {
     VBT_LEAF_ERR_KEY_RANGE = 9,
     VBT_LEAF_ERR_STREAM_OOB = 10,
     VBT_LEAF_ERR_SCI_RANGE = 11,
+    VBT_LEAF_ERR_SCI_BASIS = 12,     /* with VBT_OPEN_SCI_DENSE_BASIS: a tile / patch basis block whose active
+                                        count is not its mask's popcount, tile size above 8, or patch size or
+                                        local resolution of 1 */
     VBT_QUERY_ERR_OUTSIDE = 32,      /* point query voxel outside the volume */
     VBT_QUERY_ERR_FRAME = 33,        /* scientific query frame >= frames */
     VBT_ERR_CONTROL_INDEX = 34,      /* control index >= controlCount */
@@ -232,6 +235,19 @@ typedef struct vbt_asset_s* vbt_asset;
  * Runs on the given stream and synchronises it once at the end. */
 int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_asset* out);
 int vbt_asset_open_file(const char* path, void* stream, vbt_asset* out);
+/* Open flags.  VBT_OPEN_SCI_DENSE_BASIS: scientific leaves with dense
+ * subtype 2 (tile temporal basis) or 3 (patch temporal basis) decode as the
+ * reference encoder reconstructs its encodings
+ * (decodeDenseTileTemporalBasisAt / decodeDensePatchTemporalBasisAt,
+ * src/spatial_first_encoder.cpp:2094-2176) from the block it writes
+ * (src/spatial_first_encoder.cpp:426-458): value = coarse + that residual.
+ * Without it these subtypes decode as the reference's file decoder reads them
+ * (render/src/scientific_decode.cpp:297-331 takes every subtype but 1 for
+ * per-frame BFP data).  No counterpart in the reference's file decoder: for
+ * callers that read what the encoder measured. */
+enum { VBT_OPEN_SCI_DENSE_BASIS = 1 };
+int vbt_asset_open_memory_ex(const void* bytes, uint64_t size, void* stream, uint32_t flags, vbt_asset* out);
+int vbt_asset_open_file_ex(const char* path, void* stream, uint32_t flags, vbt_asset* out);
 void vbt_asset_close(vbt_asset a);
 
 typedef struct {

@@ -892,7 +892,183 @@ static int sci_sparse(const vbto_file* f, uint32_t base, const vbto_sci_header* 
     return VBTO_OK;
 }
 
-/* decodeScientificResidualAt, scientific_decode.cpp:297-331 */
+/* ---- dense subtypes 2 and 3: the encoder's tile / patch temporal bases ----
+ * The reference's file decoder reads these subtypes as subtype 0; the encoder
+ * reconstructs them from its in-memory encoding.  The encoding's fields come
+ * from the block the encoder writes after the coarse coefficients
+ * (appendPackedDenseTileTemporalBasis / appendPackedDensePatchTemporalBasis,
+ * src/spatial_first_encoder.cpp:426-458): tile = u64 active mask, u16 tile
+ * size, u16 active count; patch = u64 active mask, u16 patch size, u16 local
+ * resolution, u16 active count; then, when the value count is positive,
+ * temporalKeep half scales and temporalKeep packed bases of
+ * basisBytes = (max(1, values) * bits + 7) / 8 bytes, values = count * cells
+ * (encodeDenseTileTemporalBasis :1755-1773, encodeDensePatchTemporalBasis
+ * :1850-1863).  temporalKeep and bits are the leaf header's (:225-248). */
+typedef struct {
+    int patch;
+    uint64_t mask;
+    uint32_t size, localRes, count, tk, bits;
+    uint64_t values, basisBytes, scalesOff, dataOff, end; /* offsets: payload pool bytes */
+} basis_block;
+
+static int popcount_u64(uint64_t v)
+{
+    int n = 0;
+    for (; v; v &= v - 1u) ++n;
+    return n;
+}
+
+static int basis_block_read(const vbto_file* f, uint32_t base, const vbto_sci_header* dh, basis_block* b)
+{
+    const vbto_sci_header d = *dh;
+    memset(b, 0, sizeof(*b));
+    if (d.mode < 2 || d.denseSubtype < 2 || d.denseSubtype > 3) return fail("not a tile / patch basis leaf");
+    const uint64_t pool = (uint64_t)f->payloadWordCount * 4u;
+    const uint64_t blk = (uint64_t)base + 4u + 64u * (uint64_t)(d.coarseKeepMinus1 + 2u);
+    b->patch = d.denseSubtype == 3;
+    const uint64_t hdr = b->patch ? 14u : 12u;
+    if (blk + hdr > pool) return fail("dense basis block out of range");
+    uint32_t w[7];
+    for (int i = 0; i < 7; ++i)
+        if (pu16(f, (uint32_t)(blk + 2u * (uint64_t)i), &w[i])) return VBTO_ERR;
+    b->mask = (uint64_t)w[0] | ((uint64_t)w[1] << 16) | ((uint64_t)w[2] << 32) | ((uint64_t)w[3] << 48);
+    b->size = w[4];
+    b->localRes = b->patch ? w[5] : 0u;
+    b->count = b->patch ? w[6] : w[5];
+    b->tk = d.denseTemporalKeepMinus1 + 1u;
+    b->bits = d.denseQuantBits;
+    if ((uint32_t)popcount_u64(b->mask) != b->count) return fail("dense basis block: active count does not match its mask");
+    if (!b->patch && b->size > 8u) return fail("dense tile basis: tile size above the leaf size");
+    if (b->patch && b->count && (b->size == 1u || b->localRes == 1u))
+        return fail("dense patch basis: patch size or local resolution of 1");
+    const uint64_t cells = b->patch ? (uint64_t)b->localRes * b->localRes * b->localRes
+                                    : (uint64_t)b->size * b->size * b->size;
+    b->values = (uint64_t)b->count * cells;
+    b->basisBytes = ((b->values > 0u ? b->values : 1u) * b->bits + 7u) / 8u;
+    b->scalesOff = blk + hdr;
+    b->dataOff = b->scalesOff + 2u * b->tk;
+    b->end = b->values > 0u ? b->dataOff + (uint64_t)b->tk * b->basisBytes : blk + hdr;
+    if (b->end > pool) return fail("dense basis block out of range");
+    return VBTO_OK;
+}
+
+int vbto_sci_basis_extent(const vbto_file* f, uint32_t base, uint32_t headerWord, uint64_t* end)
+{
+    basis_block b;
+    vbto_sci_header d;
+    vbto_decode_sci_header(headerWord, &d);
+    if (basis_block_read(f, base, &d, &b)) return VBTO_ERR;
+    *end = b.end - base;
+    return VBTO_OK;
+}
+
+/* decodeDenseTileTemporalBasisValueAt (spatial_first_encoder.cpp:2078-2092)
+ * and its patch twin (:2119-2133): the code keeps at most 8 bits (the mask is
+ * a uint8_t) of a value read from at most two bytes */
+static int basis_value(const vbto_file* f, const basis_block* b, uint64_t ci, uint32_t k, float* out)
+{
+    const uint64_t bitOffset = ci * b->bits;
+    const uint64_t byteIndex = b->dataOff + (uint64_t)k * b->basisBytes + bitOffset / 8u;
+    const uint32_t shift = (uint32_t)(bitOffset % 8u);
+    uint32_t packed, hi, sb;
+    if (pbyte(f, (uint32_t)byteIndex, &packed)) return VBTO_ERR;
+    if (shift + b->bits > 8u) {
+        if (pbyte(f, (uint32_t)byteIndex + 1u, &hi)) return VBTO_ERR;
+        packed |= hi << 8;
+    }
+    const uint32_t q = (packed >> shift) & (((1u << b->bits) - 1u) & 0xFFu);
+    if (pu16(f, (uint32_t)(b->scalesOff + 2u * k), &sb)) return VBTO_ERR;
+    *out = decode_bfp(q, vbto_half_bits_to_float((uint16_t)sb), (int)b->bits);
+    return VBTO_OK;
+}
+
+/* decodeDenseTileTemporalBasisAt, spatial_first_encoder.cpp:2094-2114 */
+static int tile_basis(const vbto_file* f, const basis_block* b, uint32_t x, uint32_t y, uint32_t z, uint32_t t,
+                      float* out)
+{
+    *out = 0.0f;
+    if (b->count == 0u || b->size == 0u || b->values == 0u || b->basisBytes == 0u) return VBTO_OK;
+    const uint32_t s = b->size, tpa = 8u / s > 1u ? 8u / s : 1u;
+    const uint32_t tileIdx = ((z / s) * tpa + y / s) * tpa + x / s;
+    const uint64_t bit = tileIdx < 64u ? (1ull << tileIdx) : 0ull;
+    if (bit == 0u || (b->mask & bit) == 0u) return VBTO_OK;
+    const uint64_t ci = (uint64_t)popcount_u64(b->mask & (bit - 1u)) * s * s * s + ((z % s) * s + y % s) * s + x % s;
+    float residual = 0.0f;
+    for (uint32_t k = 0; k < b->tk; ++k) {
+        float v;
+        if (basis_value(f, b, ci, k, &v)) return VBTO_ERR;
+        residual += v * vbto_dct_basis((int)f->header.frames, (int)t, (int)k);
+    }
+    *out = residual;
+    return VBTO_OK;
+}
+
+static void patch_coord(uint32_t local, uint32_t size, uint32_t res, int* i0, float* w)
+{
+    const float fx = ((float)local / (float)(size - 1u)) * (float)(res - 1u);
+    int i = (int)floorf(fx);
+    if (i < 0) i = 0;
+    if (i > (int)res - 2) i = (int)res - 2;
+    *i0 = i;
+    *w = fx - (float)i;
+}
+
+/* decodeDensePatchTemporalBasisAt, spatial_first_encoder.cpp:2135-2176 */
+static int patch_basis(const vbto_file* f, const basis_block* b, uint32_t x, uint32_t y, uint32_t z, uint32_t t,
+                       float* out)
+{
+    *out = 0.0f;
+    if (b->count == 0u || b->size == 0u || b->localRes == 0u || b->values == 0u || b->basisBytes == 0u)
+        return VBTO_OK;
+    const uint32_t s = b->size, R = b->localRes;
+    const uint32_t patchIdx = ((z / s) * 2u + y / s) * 2u + x / s;
+    const uint64_t bit = patchIdx < 64u ? (1ull << patchIdx) : 0ull;
+    if (bit == 0u || (b->mask & bit) == 0u) return VBTO_OK;
+    const uint64_t first = (uint64_t)popcount_u64(b->mask & (bit - 1u)) * R * R * R;
+    int x0, y0, z0;
+    float tx, ty, tz;
+    patch_coord(x % s, s, R, &x0, &tx);
+    patch_coord(y % s, s, R, &y0, &ty);
+    patch_coord(z % s, s, R, &z0, &tz);
+    float residual = 0.0f;
+    for (uint32_t k = 0; k < b->tk; ++k) {
+        float c[8]; /* bit0 x, bit1 y, bit2 z */
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t cx = (uint64_t)(x0 + (i & 1)), cy = (uint64_t)(y0 + ((i >> 1) & 1)),
+                           cz = (uint64_t)(z0 + ((i >> 2) & 1));
+            if (basis_value(f, b, first + (cz * R + cy) * R + cx, k, &c[i])) return VBTO_ERR;
+        }
+        const float c00 = c[0] * (1.0f - tx) + c[1] * tx;
+        const float c01 = c[4] * (1.0f - tx) + c[5] * tx;
+        const float c10 = c[2] * (1.0f - tx) + c[3] * tx;
+        const float c11 = c[6] * (1.0f - tx) + c[7] * tx;
+        const float c0 = c00 * (1.0f - ty) + c10 * ty;
+        const float c1 = c01 * (1.0f - ty) + c11 * ty;
+        residual += (c0 * (1.0f - tz) + c1 * tz) * vbto_dct_basis((int)f->header.frames, (int)t, (int)k);
+    }
+    *out = residual;
+    return VBTO_OK;
+}
+
+static int basis_residual(const vbto_file* f, uint32_t base, const vbto_sci_header* d, uint32_t lx, uint32_t ly,
+                          uint32_t lz, uint32_t t, float* out)
+{
+    basis_block b;
+    *out = 0.0f;
+    if (basis_block_read(f, base, d, &b)) return VBTO_ERR;
+    return b.patch ? patch_basis(f, &b, lx, ly, lz, t, out) : tile_basis(f, &b, lx, ly, lz, t, out);
+}
+
+int vbto_sci_basis_residual(const vbto_file* f, uint32_t base, uint32_t headerWord, uint32_t lx, uint32_t ly,
+                            uint32_t lz, uint32_t t, float* out)
+{
+    vbto_sci_header d;
+    vbto_decode_sci_header(headerWord, &d);
+    return basis_residual(f, base, &d, lx, ly, lz, t, out);
+}
+
+/* decodeScientificResidualAt, scientific_decode.cpp:297-331; with
+ * f->sciDenseBasis set, dense subtypes 2 and 3 go to vbto_sci_basis_residual */
 static int sci_residual(const vbto_file* f, uint32_t base, const vbto_sci_header* d, int keep, uint32_t idx,
                         uint32_t lx, uint32_t ly, uint32_t lz, uint32_t t, float* out)
 {
@@ -900,6 +1076,8 @@ static int sci_residual(const vbto_file* f, uint32_t base, const vbto_sci_header
     case 0: *out = 0.0f; return VBTO_OK;
     case 1: return sci_sparse(f, base, d, keep, idx, t, out);
     default: {
+        if (f->sciDenseBasis && (d->denseSubtype == 2 || d->denseSubtype == 3))
+            return basis_residual(f, base, d, lx, ly, lz, t, out);
         const int R = d->mode == 2 ? 3 : 4;
         if (d->denseSubtype == 1)
             return sci_dense_basis(f, base, keep, R, (int)d->denseQuantBits, (int)d->denseTemporalKeepMinus1 + 1,

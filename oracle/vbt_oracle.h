@@ -124,6 +124,10 @@ typedef struct {
     const uint32_t* offsetsWords;  /* leafCount + 1 */
     const uint32_t* payloadWords;
     uint32_t payloadWordCount;
+    /* 0 (as loaded): dense subtypes 2 and 3 decode as the reference's file
+     * decoder reads them (as subtype 0).  1: as the encoder reconstructs its
+     * tile / patch temporal-basis blocks (vbto_sci_basis_residual). */
+    uint32_t sciDenseBasis;
 } vbto_file;
 
 /* Validates a whole file image exactly like loadVbtFile (same messages). The
@@ -231,6 +235,19 @@ int vbto_sci_coarse_control(const vbto_file* f, uint32_t payloadByteBase, int ke
 int vbto_sci_value_at(const vbto_file* f, uint32_t x, uint32_t y, uint32_t z, uint32_t t,
                       uint32_t leafIndex, float* out);
 int vbto_sci_leaf_frame(const vbto_file* f, uint32_t leafIndex, uint32_t frameIndex, float* outLeaf);
+/* The residual of a dense subtype-2 (tile) or subtype-3 (patch) temporal-basis
+ * block at local voxel (lx, ly, lz) and frame t, read as the encoder's
+ * reconstruction reads its in-memory encoding
+ * (src/spatial_first_encoder.cpp:2078-2176); the leaf's block starts at
+ * payload byte payloadByteBase + 4 + 64 * (keep + 1).  VBTO_ERR for a block
+ * that reaches past the pool or that the encoder cannot have written
+ * (active count != popcount of the mask, tile size above 8, patch size or
+ * local resolution of 1). */
+int vbto_sci_basis_residual(const vbto_file* f, uint32_t payloadByteBase, uint32_t headerWord, uint32_t lx,
+                            uint32_t ly, uint32_t lz, uint32_t t, float* out);
+/* one past the last payload byte (relative to the leaf's first byte) that
+ * vbto_sci_basis_residual reads of the leaf, or VBTO_ERR as it would fail */
+int vbto_sci_basis_extent(const vbto_file* f, uint32_t payloadByteBase, uint32_t headerWord, uint64_t* end);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,18 @@ FIXTURES = {
                                                       "--generic-patch-temporal-basis",
                                                       "--generic-patch-temporal-force"],
                        [0, 13, 39]),
+    # tile size 4 (a DenseGrid4 leaf), 3-bit codes, six temporal terms
+    "sci_tile4_mini": ("sci_dense_mini", SCI_FLAGS + ["--generic-norm-thr", "0.0005", "--generic-peak-thr", "0.0005",
+                                                      "--generic-tile-temporal-basis", "--generic-tile-temporal-force",
+                                                      "--generic-tile-size", "4", "--generic-tile-bits", "3",
+                                                      "--generic-tile-temporal-keep", "6"],
+                       [0, 13, 39]),
+    # patch basis with 2-bit codes (the four-level table) and seven terms
+    "sci_patch_b2_mini": ("sci_dense_mini", SCI_FLAGS + ["--generic-norm-thr", "0.0005", "--generic-peak-thr",
+                                                         "0.0005", "--generic-patch-temporal-basis",
+                                                         "--generic-patch-temporal-force", "--generic-multipatch-bits", "2",
+                                                         "--generic-patch-temporal-keep", "7"],
+                          [0, 13, 39]),
 }
 
 

@@ -25,3 +25,9 @@ def vbtp(name: str) -> bytes:
 
 def expected(name: str):
     return dict(np.load(os.path.join(GOLDEN, name + ".expected.npz"), allow_pickle=False))
+
+
+def basis(name: str):
+    """Expected outputs with dense subtypes 2 / 3 decoded as the encoder
+    reconstructs them (tests/golden/make_basis_golden.py)."""
+    return dict(np.load(os.path.join(GOLDEN, name + ".basis.npz"), allow_pickle=False))

@@ -54,7 +54,8 @@ class FileHeader(ctypes.Structure):
 
 class VFile(ctypes.Structure):
     _fields_ = [("header", FileHeader), ("coarseAcScales", ctypes.c_void_p), ("offsetsWords", ctypes.c_void_p),
-                ("payloadWords", ctypes.c_void_p), ("payloadWordCount", ctypes.c_uint32)]
+                ("payloadWords", ctypes.c_void_p), ("payloadWordCount", ctypes.c_uint32),
+                ("sciDenseBasis", ctypes.c_uint32)]
 
 
 class Compose(ctypes.Structure):
@@ -114,6 +115,10 @@ def oracle():
         L.vbto_dct_basis.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.vbto_sci_value_at.argtypes = [ctypes.POINTER(VFile)] + [ctypes.c_uint32] * 5 + [ctypes.POINTER(ctypes.c_float)]
         L.vbto_sci_leaf_frame.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+        L.vbto_sci_basis_residual.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32, ctypes.c_uint32] + \
+            [ctypes.c_uint32] * 4 + [ctypes.POINTER(ctypes.c_float)]
+        L.vbto_sci_basis_extent.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.POINTER(ctypes.c_uint64)]
         L.vbto_decode_render_leaf_frame.argtypes = [ctypes.POINTER(VFile), ctypes.c_uint32, ctypes.c_int,
                                                     ctypes.POINTER(Compose), ctypes.c_void_p]
         L.vbto_cycles_pack_blob.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
@@ -170,12 +175,13 @@ def compose(sign_rule=0, background=0.0, shell_empty_scale=1.0, shell_empty_max=
 class OracleFile:
     """A VBTPACK4 image validated by the oracle loader (keeps the bytes alive)."""
 
-    def __init__(self, data: bytes):
+    def __init__(self, data: bytes, dense_basis: bool = False):
         self.buf = np.frombuffer(bytearray(data), dtype=np.uint8)  # 4-byte aligned numpy allocation
         self.vf = VFile()
         rc = oracle().vbto_load_file_memory(self.buf.ctypes.data, len(data), ctypes.byref(self.vf))
         if rc != 0:
             raise ValueError(last_error())
+        self.vf.sciDenseBasis = 1 if dense_basis else 0
         h = self.vf.header
         self.header = h
         n_scale = h.coarseAcScaleCount

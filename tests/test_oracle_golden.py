@@ -9,7 +9,7 @@ import golden_io
 import oracle_py
 
 RENDER = ["smoke64", "liquid_mini"]
-SCI = ["sci_mini", "sci_dense_mini", "sci_tile_mini"]
+SCI = ["sci_mini", "sci_dense_mini", "sci_tile_mini", "sci_tile4_mini"]
 
 
 @pytest.mark.parametrize("name", RENDER)
@@ -72,17 +72,19 @@ def test_dense_subtypes_2_and_3_follow_the_reference_decoder():
     fixture's reads run past the payload pool, where the reference throws and
     the oracle reports an out-of-range read."""
     m = golden_io.manifest()
-    for name, sub in (("sci_tile_mini", 2), ("sci_patch_mini", 3)):
+    for name, sub, mode in (("sci_tile_mini", 2, 2), ("sci_tile4_mini", 2, 3), ("sci_patch_mini", 3, 2),
+                            ("sci_patch_b2_mini", 3, 2)):
         f = oracle_py.OracleFile(golden_io.vbtp(name))
         words = f.payload.view(np.uint32)
         hdr = words[f.offsets[:-1][f.offsets[1:] > f.offsets[:-1]]]
         assert ((hdr & 3) >= 2).all() and (((hdr >> 12) & 7) == sub).all(), name
-        assert m[name]["modes"][2] == 48
-    exp = golden_io.expected("sci_patch_mini")
-    assert "range_check" in str(exp["error"])
-    f = oracle_py.OracleFile(golden_io.vbtp("sci_patch_mini"))
-    for i, t in enumerate(exp["frames"]):  # frames the reference decoded before throwing
-        np.testing.assert_array_equal(f.frame(int(t)).view(np.uint32), exp["dense"][i].view(np.uint32))
-    with pytest.raises(ValueError, match="out of range"):
-        f.frame(int(exp["error_frame"]))
+        assert m[name]["modes"][mode] == 48
+    for name in ("sci_patch_mini", "sci_patch_b2_mini"):
+        exp = golden_io.expected(name)
+        assert "range_check" in str(exp["error"])
+        f = oracle_py.OracleFile(golden_io.vbtp(name))
+        for i, t in enumerate(exp["frames"]):  # frames the reference decoded before throwing
+            np.testing.assert_array_equal(f.frame(int(t)).view(np.uint32), exp["dense"][i].view(np.uint32))
+        with pytest.raises(ValueError, match="out of range"):
+            f.frame(int(exp["error_frame"]))
 

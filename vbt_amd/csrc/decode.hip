@@ -273,6 +273,26 @@ __device__ inline uint64_t sci_read_end(const AssetDev& a, const DevLeaf& d)
     return end;
 }
 
+// with ASSET_SCI_DENSE_BASIS, a subtype-2/3 leaf: one past the last payload
+// byte its decode reads, and LEAF_ERR_SCI_BASIS for a block the encoder
+// cannot have written (active count != popcount of the mask, tile size above
+// 8, patch size or local resolution of 1 with active patches), whose decode
+// would index outside the block
+__device__ inline uint32_t sci_basis_check(const AssetDev& a, const DevLeaf& d, const SciHeader& h, uint64_t& end)
+{
+    const uint64_t base = (uint64_t)d.word * 4u;
+    const uint64_t hdr = base + 4u + 64u * (h.keep + 1u) + (h.subtype == 3u ? 14u : 12u);
+    end = hdr;
+    if (hdr > a.payloadBytes) return LEAF_ERR_SCI_RANGE;
+    const SciBasis s = sci_basis(Bytes{a.payload + base}, h);
+    if (s.values > 0u) end = base + s.dataOff + (uint64_t)h.tk * s.basisBytes;
+    if ((uint32_t)__popcll(s.mask) != s.count) return LEAF_ERR_SCI_BASIS;
+    if (!s.patch && s.size > 8u) return LEAF_ERR_SCI_BASIS;
+    if (s.patch && s.count != 0u && (s.size == 1u || s.res == 1u)) return LEAF_ERR_SCI_BASIS;
+    if (end > a.payloadBytes || end - base >= 0xFFFFFFFFull) return LEAF_ERR_SCI_RANGE;
+    return LEAF_OK;
+}
+
 // validation: one workgroup per active leaf
 __global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, uint32_t* status)
 {
@@ -290,8 +310,14 @@ __global__ void __launch_bounds__(256) k_validate(AssetDev a, DevLeaf* leaves, u
         if (threadIdx.x == 0) {
             uint32_t st = LEAF_OK;
             if (h.keep > 1u && h.keep - 1u > a.scaleCount) st = LEAF_ERR_SCI_RANGE;
-            const uint64_t end = sci_read_end(a, d);
-            if (end > a.payloadBytes) st = LEAF_ERR_SCI_RANGE;
+            uint64_t end;
+            if (sci_basis_leaf(a, h)) {
+                const uint32_t bs = sci_basis_check(a, d, h, end);
+                if (st == LEAF_OK) st = bs;
+            } else {
+                end = sci_read_end(a, d);
+                if (end > a.payloadBytes) st = LEAF_ERR_SCI_RANGE;
+            }
             const uint64_t rel = end - (uint64_t)d.word * 4u;
             d.sciEnd = rel < 0xFFFFFFFFull ? (uint32_t)rel : 0xFFFFFFFFu;
             sErr = st;
@@ -1234,6 +1260,7 @@ struct vbt_asset_s {
     uint64_t controlsPerFrame = 0, payloadBytes = 0, deviceBytes = 0, fast = 0;
     uint64_t modeCounts[4] = {0, 0, 0, 0};
     uint32_t sciKeepMax = 1;        // scientific: largest DCT term count of a leaf
+    uint32_t flags = 0;             // ASSET_* (the VBT_OPEN_* flags it was opened with)
     std::string badMessage; // first poisoned leaf and its reference exception text
 
     AssetDev dev() const
@@ -1259,6 +1286,7 @@ struct vbt_asset_s {
         a.slotPlan = d_slotPlan;
         a.groupRange = d_groupRange;
         a.controlsPerFrame = controlsPerFrame;
+        a.flags = flags;
         return a;
     }
 };
@@ -1461,7 +1489,13 @@ int vbt_device_count(int* n)
 
 int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_asset* out)
 {
+    return vbt_asset_open_memory_ex(bytes, size, stream, 0u, out);
+}
+
+int vbt_asset_open_memory_ex(const void* bytes, uint64_t size, void* stream, uint32_t flags, vbt_asset* out)
+{
     *out = nullptr;
+    if (flags & ~(uint32_t)VBT_OPEN_SCI_DENSE_BASIS) return vbt_fail(VBT_ERR_INVALID, "unknown asset open flags");
     vbt_file_header h;
     int rc = vbt_validate_file(bytes, size, &h);
     if (rc) return rc;
@@ -1470,6 +1504,7 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
     hipStream_t s = S(stream);
     vbt_asset a = new vbt_asset_s();
     a->h = h;
+    a->flags = (flags & VBT_OPEN_SCI_DENSE_BASIS) ? ASSET_SCI_DENSE_BASIS : 0u;
     HIPCHK(hipGetDevice(&a->device));
     const uint8_t* base = (const uint8_t*)bytes;
     const uint64_t scaleOff = 52, offOff = 52 + 4ull * h.coarseAcScaleCount;
@@ -1584,6 +1619,11 @@ int vbt_asset_open_memory(const void* bytes, uint64_t size, void* stream, vbt_as
 
 int vbt_asset_open_file(const char* path, void* stream, vbt_asset* out)
 {
+    return vbt_asset_open_file_ex(path, stream, 0u, out);
+}
+
+int vbt_asset_open_file_ex(const char* path, void* stream, uint32_t flags, vbt_asset* out)
+{
     std::ifstream in(path, std::ios::binary | std::ios::ate);
     if (!in) return vbt_fail(VBT_ERR_INVALID, std::string("Failed to open file: ") + path);
     const std::streamsize n = in.tellg();
@@ -1591,7 +1631,7 @@ int vbt_asset_open_file(const char* path, void* stream, vbt_asset* out)
     in.seekg(0);
     in.read((char*)buf.data(), n);
     if (!in) return vbt_fail(VBT_ERR_INVALID, std::string("Failed to read file: ") + path);
-    return vbt_asset_open_memory(buf.data(), (uint64_t)n, stream, out);
+    return vbt_asset_open_memory_ex(buf.data(), (uint64_t)n, stream, flags, out);
 }
 
 void vbt_asset_close(vbt_asset a) { free_asset(a); }

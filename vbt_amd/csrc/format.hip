@@ -439,6 +439,7 @@ const char* vbt_status_message(int32_t s)
     case VBT_LEAF_ERR_KEY_RANGE: return "render temporal keyframe range out of bounds";
     case VBT_LEAF_ERR_STREAM_OOB: return "render temporal stream decode out of bounds";
     case VBT_LEAF_ERR_SCI_RANGE: return "scientific payload read outside the payload pool";
+    case VBT_LEAF_ERR_SCI_BASIS: return "scientific dense basis block is inconsistent";
     case VBT_QUERY_ERR_OUTSIDE: return "query voxel outside the volume";
     case VBT_QUERY_ERR_FRAME: return "scientific query frame out of range";
     case VBT_ERR_CONTROL_INDEX: return "render temporal control index out of range";

@@ -43,6 +43,7 @@ enum : uint32_t {
     LEAF_ERR_KEY_RANGE = 9,        // "render temporal keyframe range out of bounds"
     LEAF_ERR_STREAM_OOB = 10,      // "render temporal stream decode out of bounds"
     LEAF_ERR_SCI_RANGE = 11,       // scientific payload read outside the payload pool
+    LEAF_ERR_SCI_BASIS = 12,       // tile / patch basis block the encoder cannot have written
     LEAF_ERR_COUNT
 };
 
@@ -416,6 +417,38 @@ VBT_HD void sci_dense_coord(uint32_t x, int R, int& i0, float& t)
     if (i > R - 2) i = R - 2;
     i0 = i;
     t = f - (float)i;
+}
+
+// ---- dense subtypes 2 (tile) and 3 (patch) as the encoder reconstructs them
+// The block after the coarse coefficients (appendPackedDenseTileTemporalBasis /
+// appendPackedDensePatchTemporalBasis, src/spatial_first_encoder.cpp:426-458):
+// tile = u64 active mask, u16 tile size, u16 active count; patch = u64 active
+// mask, u16 patch size, u16 local resolution, u16 active count; then, when the
+// value count (active count x cells) is positive, tk half scales and tk packed
+// bases of (max(1, values) * bits + 7) / 8 bytes; tk and bits are the leaf
+// header's.
+struct SciBasis {
+    uint64_t mask, values, basisBytes;
+    uint32_t patch, size, res, count;
+    uint32_t scalesOff, dataOff; // from the leaf's first byte
+};
+
+template <class B>
+VBT_HD SciBasis sci_basis(const B& b, const SciHeader& h)
+{
+    SciBasis s;
+    const uint32_t blk = 4u + 64u * (h.keep + 1u);
+    s.patch = h.subtype == 3u ? 1u : 0u;
+    s.mask = (uint64_t)b.u32(blk) | ((uint64_t)b.u32(blk + 4u) << 32);
+    s.size = b.u16(blk + 8u);
+    s.res = s.patch ? b.u16(blk + 10u) : 0u;
+    s.count = b.u16(blk + (s.patch ? 12u : 10u));
+    const uint64_t e = s.patch ? s.res : s.size;
+    s.values = (uint64_t)s.count * e * e * e;
+    s.basisBytes = ((s.values > 0u ? s.values : 1u) * h.bits + 7u) / 8u;
+    s.scalesOff = blk + (s.patch ? 14u : 12u);
+    s.dataOff = s.scalesOff + 2u * h.tk;
+    return s;
 }
 
 // decodeBfp (render/src/scientific_decode.cpp:61-70)

@@ -55,9 +55,19 @@ struct AssetDev {
     const uint2* slotPlan;      // render: per packed-cache slot, where and how its records are swept
     const uint2* groupRange;    // render: per CACHE_GROUP-slot group, byte range of its swept records
     uint32_t W, H, D, F, L, A, profile, scaleCount;
+    uint32_t flags;             // ASSET_* open flags
     uint64_t payloadBytes;
     uint64_t controlsPerFrame;
 };
+
+// scientific dense subtypes 2 and 3 decode as the encoder reconstructs them
+// (the open flag VBT_OPEN_SCI_DENSE_BASIS)
+constexpr uint32_t ASSET_SCI_DENSE_BASIS = 1u;
+
+__host__ __device__ inline bool sci_basis_leaf(const AssetDev& a, const SciHeader& h)
+{
+    return (a.flags & ASSET_SCI_DENSE_BASIS) != 0u && h.mode >= 2u && (h.subtype == 2u || h.subtype == 3u);
+}
 
 struct ComposeDev {
     int signRule;
@@ -113,10 +123,83 @@ __device__ inline uint32_t bfp_code(const B& b, uint32_t blockBase, uint32_t ci,
     return (packed >> shift) & ((1u << bits) - 1u);
 }
 
+// basis term k of value ci of a tile / patch block
+// (decodeDenseTileTemporalBasisValueAt, src/spatial_first_encoder.cpp:2078-2092,
+// and its patch twin :2119-2133): the code keeps at most 8 bits (the mask is
+// a uint8_t) of a value read from at most two bytes
+template <class B>
+__device__ inline float sci_basis_value(const B& b, const SciBasis& s, uint32_t bits, uint64_t ci, uint32_t k)
+{
+    const uint64_t bit = ci * bits;
+    const uint32_t byteIndex = (uint32_t)(s.dataOff + (uint64_t)k * s.basisBytes + bit / 8u);
+    const uint32_t shift = (uint32_t)(bit % 8u);
+    uint32_t packed = b.u8(byteIndex);
+    if (shift + bits > 8u) packed |= b.u8(byteIndex + 1u) << 8;
+    const uint32_t q = (packed >> shift) & (((1u << bits) - 1u) & 0xFFu);
+    return decode_bfp(q, half_to_float(b.u16(s.scalesOff + 2u * k)), (int)bits);
+}
+
+// patch lattice of decodeDensePatchTemporalBasisAt
+// (src/spatial_first_encoder.cpp:2147-2161)
+__device__ inline void sci_patch_coord(uint32_t local, uint32_t size, uint32_t res, int& i0, float& w)
+{
+    const float f = ((float)local / (float)(size - 1u)) * (float)(res - 1u);
+    int i = (int)floorf(f);
+    if (i < 0) i = 0;
+    if (i > (int)res - 2) i = (int)res - 2;
+    i0 = i;
+    w = f - (float)i;
+}
+
+// the residual of a tile (decodeDenseTileTemporalBasisAt,
+// src/spatial_first_encoder.cpp:2094-2114) or patch block
+// (decodeDensePatchTemporalBasisAt, :2135-2176).  Validation at open keeps
+// every read inside the block: active count == popcount(mask), tile size <= 8,
+// no patch size or resolution of 1 with active patches.
+template <class B>
+__device__ inline float sci_basis_residual(const AssetDev& a, const B& b, const SciHeader& h, uint32_t x,
+                                           uint32_t y, uint32_t z, uint32_t t)
+{
+    const SciBasis s = sci_basis(b, h);
+    if (s.count == 0u || s.size == 0u || s.values == 0u || s.basisBytes == 0u) return 0.0f;
+    const float* basis = a.basis + (size_t)t * 64u;
+    float residual = 0.0f;
+    if (!s.patch) {
+        const uint32_t n = s.size, tpa = 8u / n > 1u ? 8u / n : 1u;
+        const uint32_t tile = ((z / n) * tpa + y / n) * tpa + x / n;
+        if (tile >= 64u || ((s.mask >> tile) & 1u) == 0u) return 0.0f;
+        const uint64_t ci = (uint64_t)__popcll(s.mask & ((1ull << tile) - 1u)) * n * n * n +
+                            ((z % n) * n + y % n) * n + x % n;
+        for (uint32_t k = 0; k < h.tk; ++k) residual += sci_basis_value(b, s, h.bits, ci, k) * basis[k];
+        return residual;
+    }
+    if (s.res == 0u) return 0.0f;
+    const uint32_t n = s.size, R = s.res;
+    const uint32_t patch = ((z / n) * 2u + y / n) * 2u + x / n;
+    if (patch >= 64u || ((s.mask >> patch) & 1u) == 0u) return 0.0f;
+    const uint64_t first = (uint64_t)__popcll(s.mask & ((1ull << patch) - 1u)) * R * R * R;
+    int x0, y0, z0;
+    float tx, ty, tz;
+    sci_patch_coord(x % n, n, R, x0, tx);
+    sci_patch_coord(y % n, n, R, y0, ty);
+    sci_patch_coord(z % n, n, R, z0, tz);
+    for (uint32_t k = 0; k < h.tk; ++k) {
+        float c[8];
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t cx = (uint64_t)(x0 + (i & 1)), cy = (uint64_t)(y0 + ((i >> 1) & 1)),
+                           cz = (uint64_t)(z0 + ((i >> 2) & 1));
+            c[i] = sci_basis_value(b, s, h.bits, first + (cz * R + cy) * R + cx, k);
+        }
+        residual += trilinear(c, tx, ty, tz) * basis[k];
+    }
+    return residual;
+}
+
 template <class B>
 __device__ inline float sci_dense_value(const AssetDev& a, const B& b, const SciHeader& h, uint32_t lx, uint32_t ly,
                                         uint32_t lz, uint32_t t)
 {
+    if (sci_basis_leaf(a, h)) return sci_basis_residual(a, b, h, lx, ly, lz, t);
     const int R = h.mode == 2u ? 3 : 4;
     const uint32_t bits = h.bits;
     const uint32_t vals = (uint32_t)(R * R * R);

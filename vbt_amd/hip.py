@@ -143,6 +143,8 @@ def lib():
         L.vbt_device_count.argtypes = [ctypes.POINTER(c_int)]
         L.vbt_asset_open_memory.argtypes = [c_void_p, c_u64, c_void_p, ctypes.POINTER(c_void_p)]
         L.vbt_asset_open_file.argtypes = [ctypes.c_char_p, c_void_p, ctypes.POINTER(c_void_p)]
+        L.vbt_asset_open_memory_ex.argtypes = [c_void_p, c_u64, c_void_p, ctypes.c_uint32, ctypes.POINTER(c_void_p)]
+        L.vbt_asset_open_file_ex.argtypes = [ctypes.c_char_p, c_void_p, ctypes.c_uint32, ctypes.POINTER(c_void_p)]
         L.vbt_asset_close.argtypes = [c_void_p]
         L.vbt_asset_close.restype = None
         L.vbt_asset_info_get.argtypes = [c_void_p, ctypes.POINTER(AssetInfo)]
@@ -363,24 +365,34 @@ class Event:
             pass
 
 
-class Asset:
-    """A VBTPACK4 asset resident in HBM with its device page table."""
+OPEN_SCI_DENSE_BASIS = 1  # VBT_OPEN_SCI_DENSE_BASIS
 
-    def __init__(self, data: bytes, stream: Stream | None = None):
+
+class Asset:
+    """A VBTPACK4 asset resident in HBM with its device page table.
+
+    dense_basis: scientific dense subtypes 2 (tile) and 3 (patch) decode as
+    the reference encoder reconstructs them instead of as its file decoder
+    reads them (VBT_OPEN_SCI_DENSE_BASIS)."""
+
+    def __init__(self, data: bytes, stream: Stream | None = None, dense_basis: bool = False):
         self.stream = stream or Stream()
         self._buf = np.frombuffer(bytearray(data), dtype=np.uint8)
         self.h = c_void_p()
-        check(lib().vbt_asset_open_memory(self._buf.ctypes.data, len(data), self.stream.h, ctypes.byref(self.h)))
+        flags = OPEN_SCI_DENSE_BASIS if dense_basis else 0
+        check(lib().vbt_asset_open_memory_ex(self._buf.ctypes.data, len(data), self.stream.h, flags,
+                                             ctypes.byref(self.h)))
         self._buf = None
         self.info = AssetInfo()
         check(lib().vbt_asset_info_get(self.h, ctypes.byref(self.info)))
 
     @classmethod
-    def open_file(cls, path: str, stream: Stream | None = None):
+    def open_file(cls, path: str, stream: Stream | None = None, dense_basis: bool = False):
         self = cls.__new__(cls)
         self.stream = stream or Stream()
         self.h = c_void_p()
-        check(lib().vbt_asset_open_file(path.encode(), self.stream.h, ctypes.byref(self.h)))
+        flags = OPEN_SCI_DENSE_BASIS if dense_basis else 0
+        check(lib().vbt_asset_open_file_ex(path.encode(), self.stream.h, flags, ctypes.byref(self.h)))
         self.info = AssetInfo()
         check(lib().vbt_asset_info_get(self.h, ctypes.byref(self.info)))
         return self
