@@ -56,23 +56,25 @@ private:
     size_t n_ = 0;
 };
 
-// A VBTPACK4 asset resident in HBM, with its device page table.
+// A VBTPACK4 asset resident in HBM, with its device page table.  flags:
+// VBT_OPEN_* (VBT_OPEN_SCI_DENSE_BASIS: scientific dense subtypes 2 / 3 as
+// the encoder reconstructs them).
 class Asset {
 public:
-    Asset(const void* bytes, uint64_t size, void* stream = nullptr) : stream_(stream)
+    Asset(const void* bytes, uint64_t size, void* stream = nullptr, uint32_t flags = 0) : stream_(stream)
     {
-        check(vbt_asset_open_memory(bytes, size, stream, &a_));
+        check(vbt_asset_open_memory_ex(bytes, size, stream, flags, &a_));
         check(vbt_asset_info_get(a_, &info_));
     }
-    explicit Asset(const vbt::render::VbtFile& f, void* stream = nullptr) : stream_(stream)
+    explicit Asset(const vbt::render::VbtFile& f, void* stream = nullptr, uint32_t flags = 0) : stream_(stream)
     {
         const std::vector<uint8_t> img = vbt::render::hip_detail::image(f);
-        check(vbt_asset_open_memory(img.data(), img.size(), stream, &a_));
+        check(vbt_asset_open_memory_ex(img.data(), img.size(), stream, flags, &a_));
         check(vbt_asset_info_get(a_, &info_));
     }
-    explicit Asset(const std::string& path, void* stream = nullptr) : stream_(stream)
+    explicit Asset(const std::string& path, void* stream = nullptr, uint32_t flags = 0) : stream_(stream)
     {
-        check(vbt_asset_open_file(path.c_str(), stream, &a_));
+        check(vbt_asset_open_file_ex(path.c_str(), stream, flags, &a_));
         check(vbt_asset_info_get(a_, &info_));
     }
     ~Asset() { vbt_asset_close(a_); }
