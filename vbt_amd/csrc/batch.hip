@@ -133,9 +133,16 @@ constexpr uint32_t BATCH_STAGE = 8u;
 template <uint32_t BLOCK, uint32_t WINDOW>
 __device__ __forceinline__ void batch_item(const BatchArgs& A, uint32_t it, uint32_t* lds, uint32_t* sRed)
 {
-    const uint2 e = A.entries[it / A.ranges];
+    const uint2 e = invariant_u2(A.entries, it / A.ranges);
     const uint32_t r = it % A.ranges;
-    const BatchJob& J = A.jobs[e.x];
+    BatchJob J; // the job table is invariant while the kernel runs: scalar loads
+    {
+        static_assert(sizeof(BatchJob) % 4 == 0, "job table words");
+        const uint32_t* src = (const uint32_t*)(A.jobs + e.x);
+        uint32_t* dst = (uint32_t*)&J;
+#pragma unroll
+        for (uint32_t i = 0; i < sizeof(BatchJob) / 4; ++i) dst[i] = invariant_u32(src, i);
+    }
     const AssetDev a = J.a;
     const int f0 = J.t0 + (int)r * J.per;
     const int n = min(J.per, J.t0 + J.nt - f0);
@@ -143,7 +150,7 @@ __device__ __forceinline__ void batch_item(const BatchArgs& A, uint32_t it, uint
     const uint64_t total = a.controlsPerFrame;
     if (J.kind == JOB_PACKED) {
         float* out = J.out + (uint64_t)(f0 - J.t0) * total;
-        const uint32_t g0 = J.tiles[e.y], ng = J.tiles[e.y + 1u] - g0;
+        const uint32_t g0 = invariant_u32(J.tiles, e.y), ng = invariant_u32(J.tiles, e.y + 1u) - g0;
         if (n <= CACHE_EXACT_MAX_FRAMES) exact_tile(a, g0, ng, f0, n, out, total, J.err);
         else cache_plan_tile<BLOCK, WINDOW, BATCH_STAGE>(a, g0, ng, f0, n, out, total, J.err, lds, sRed);
     } else {

@@ -1051,6 +1051,21 @@ constexpr uint32_t PLAN_SLOW = 0u, PLAN_FP16 = 1u, PLAN_FP16_NAN = 2u, PLAN_FP32
 __host__ __device__ inline bool plan_swept(uint32_t kind) { return kind >= PLAN_FP16 && kind <= PLAN_FP32; }
 __host__ __device__ inline uint32_t plan_kind(uint2 p) { return (p.y >> 8) & 7u; }
 
+// Wave-uniform loads from tables that no kernel writes while it runs (tile
+// lists, group ranges, batch work items: written at asset open or before the
+// launch).  Read through the constant address space they stay scalar loads
+// even where the kernel has stored before them (a work-queue kernel's next
+// item), which the compiler otherwise turns into vector loads.
+__device__ __forceinline__ uint32_t invariant_u32(const uint32_t* p, uint64_t i)
+{
+    return ((const __attribute__((address_space(4))) uint32_t*)p)[i];
+}
+__device__ __forceinline__ uint2 invariant_u2(const uint2* p, uint64_t i)
+{
+    const uint64_t v = ((const __attribute__((address_space(4))) uint64_t*)p)[i];
+    return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+}
+
 __device__ inline uint2 slot_plan(const AssetDev& a, uint64_t g)
 {
     uint32_t c = a.groupLeaf[g / CACHE_GROUP];
@@ -1139,7 +1154,7 @@ __device__ __forceinline__ void cache_plan_tile(const AssetDev& a, uint32_t g0, 
 #pragma unroll
     for (uint32_t w = 0; w < NW; ++w) {
         if (w < ng) {
-            const uint2 r = a.groupRange[g0 + w];
+            const uint2 r = invariant_u2(a.groupRange, g0 + w);
             tlo = min(tlo, r.x);
             thi = max(thi, r.y);
         }
