@@ -743,6 +743,19 @@ def main():
         print(f"[bench] ceilings: fill {ceil['fill_gbs']} GB/s, copy {ceil['copy_gbs']} GB/s", file=sys.stderr,
               flush=True)
 
+    # the build's own memory traffic without its decode, same tiles and
+    # stores: writes alone, and with each tile's keyframe records staged
+    layout = None
+    if not args.no_ceilings:
+        ms_w = timed(hip, s, lambda: asset.cache_ceiling_into(out, nt, best, reads=False, stream=s), 10)
+        ms_rw = timed(hip, s, lambda: asset.cache_ceiling_into(out, nt, best, reads=True, stream=s), 10)
+        layout = {"write_only_ms": round(ms_w, 4), "stage_and_write_ms": round(ms_rw, 4),
+                  "build_ms": round(ms_step, 4), "build_over_stage_and_write": round(ms_step / ms_rw, 3),
+                  "stage_and_write_frac_of_peak": round(byt["total"] / (ms_rw * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        print(f"[bench] layout ceiling ({shapes[best][0]} x {shapes[best][1]} tiles, no decode): writes only "
+              f"{ms_w:.4f} ms, records staged + writes {ms_rw:.4f} ms; the build takes {ms_step / ms_rw:.3f} x "
+              f"the latter", file=sys.stderr, flush=True)
+
     if args.copy_probe_mb:
         nb = args.copy_probe_mb << 20
         src = hip.DeviceArray(nb // 4, np.float32, s)
@@ -800,7 +813,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, kname),
                      "bytes_per_launch": byt["total"],
                      "kernel": kname,
-                     "device_ceilings_gbs": ceil},
+                     "device_ceilings_gbs": ceil, "layout_ceiling": layout},
         "queries": {"random_65536": qres, "random_256": small, "patterns_65536": patterns},
         "single_frame": single,
     }

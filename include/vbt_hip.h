@@ -291,6 +291,13 @@ int vbt_cache_tile_shape(int tile, uint32_t* block, uint32_t* window);
 int vbt_asset_set_cache_tile(vbt_asset a, int tile);
 int vbt_asset_get_cache_tile(vbt_asset a, int* tile);
 int vbt_asset_autotune(vbt_asset a, int nt, int reps, float* msPerTile, int* best, void* stream);
+/* The memory traffic of a packed frame-cache build of nt frames without its
+ * decode, for timing the layout's ceiling on this render asset: the same
+ * tiles of shape `tile`, the same keyframe record bytes staged into LDS
+ * (reads != 0; none when 0) and the same output stores.  d_out (nt x
+ * controlsPerFrame floats) receives placeholder values, not a frame cache.
+ * No counterpart in the reference (a measurement aid, like vbt_device_copy). */
+int vbt_asset_cache_ceiling(vbt_asset a, int nt, int tile, int reads, float* d_out, void* stream);
 
 /* The tuning table: tile-shape picks keyed by the asset statistics the best
  * shape depends on, one table per process, kept in a JSON file between runs.

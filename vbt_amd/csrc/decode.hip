@@ -437,6 +437,54 @@ __global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32
     }
 }
 
+// The memory traffic of a packed frame-cache build without its decode: the
+// same tiles, each tile's keyframe record span staged into LDS the same way
+// (reads != 0; tiles holding a slot that takes the bracket search stage
+// nothing) and the same write-back stores of every frame's row segment, with
+// a value taken from the staged words so that the reads stay live.  It times
+// the ceiling of the build for this asset and layout; what it writes is not a
+// frame cache.
+template <uint32_t BLOCK, uint32_t WINDOW>
+__global__ void __launch_bounds__(BLOCK) k_cache_ceiling(AssetDev a, const uint32_t* tileGroup, int nt, int reads,
+                                                         float* out, uint64_t total)
+{
+    __shared__ uint32_t lds[WINDOW / 4];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t g0 = tileGroup[blockIdx.x], ng = tileGroup[blockIdx.x + 1u] - g0;
+    const uint64_t g = (uint64_t)g0 * CACHE_GROUP + tid;
+    const bool live = tid < ng * CACHE_GROUP && g < total;
+    uint32_t v = tid;
+    if (reads) {
+        uint32_t tlo = 0xFFFFFFFFu, thi = 0u;
+#pragma unroll
+        for (uint32_t w = 0; w < BLOCK / CACHE_GROUP; ++w) {
+            if (w < ng) {
+                const uint2 r = invariant_u2(a.groupRange, g0 + w);
+                tlo = min(tlo, r.x);
+                thi = max(thi, r.y);
+            }
+        }
+        if (tlo != 0xFFFFFFFFu && thi != 0xFFFFFFFFu) {
+            const uint32_t w0 = tlo / 4u, w1 = (thi + 3u) / 4u;
+            uint32_t acc = 0u;
+            for (uint32_t p = w0; p < w1; p += WINDOW / 4u) {
+                const uint32_t nw = min(w1 - p, WINDOW / 4u);
+                stage_window<BLOCK, 32u>(a.payload, p, nw, lds, tid);
+                __syncthreads();
+                acc += lds[tid % nw];
+                __syncthreads();
+            }
+            v = acc;
+        }
+    }
+    if (!live) return;
+    char* ob = (char*)out;
+    const uint64_t strideBytes = total * 4u;
+    for (int f = 0; f < nt; ++f, ob += strideBytes)
+        __builtin_amdgcn_raw_buffer_store_b32(v + (uint32_t)f, frame_rsrc(ob, (uint32_t)strideBytes), (int)(g * 4u), 0,
+                                              STORE_WB);
+}
+
 // Few frames: one thread per cache slot, the reference bracket search per
 // frame straight from the payload (about three keyframe records per frame
 // instead of the slot's whole keyframe stream).
@@ -664,6 +712,28 @@ void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* tileGroup
         k_cache_render<LAYOUT, B, W><<<LAYOUT == VBT_CACHE_PACKED ? nTiles : blocks_for(total, B), B, 0, s>>>( \
             ad, tileGroup, t0, nt, out, total, err);                                                        \
         break;
+        VBT_TILE(0, 64, 8192)
+        VBT_TILE(1, 64, 16384)
+        VBT_TILE(2, 128, 16384)
+        VBT_TILE(3, 128, 32768)
+        VBT_TILE(4, 256, 32768)
+        VBT_TILE(5, 256, 49152)
+        VBT_TILE(6, 64, 4096)
+        VBT_TILE(7, 128, 8192)
+        VBT_TILE(8, 256, 16384)
+        VBT_TILE(9, 64, 12288)
+#undef VBT_TILE
+    default:
+        break;
+    }
+}
+
+void launch_cache_ceiling(int tile, const AssetDev& ad, const uint32_t* tileGroup, uint32_t nTiles, int nt, int reads,
+                          float* out, uint64_t total, hipStream_t s)
+{
+    switch (tile) {
+#define VBT_TILE(i, B, W)                                                                                  \
+    case i: k_cache_ceiling<B, W><<<nTiles, B, 0, s>>>(ad, tileGroup, nt, reads, out, total); break;
         VBT_TILE(0, 64, 8192)
         VBT_TILE(1, 64, 16384)
         VBT_TILE(2, 128, 16384)
@@ -1711,6 +1781,20 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
     }
     HIPCHK(hipGetLastError());
     if (a->bad) return report_bad(a, "frame cache");
+    return VBT_OK;
+}
+
+int vbt_asset_cache_ceiling(vbt_asset a, int nt, int tile, int reads, float* d_out, void* stream)
+{
+    if (!a || !d_out || nt <= 0 || (uint64_t)nt > a->h.frames)
+        return vbt_fail(VBT_ERR_INVALID, "invalid frame-cache ceiling request");
+    if (tile < 0 || tile >= CACHE_TILE_COUNT) return vbt_fail(VBT_ERR_INVALID, "unknown tile shape");
+    if (a->h.profileType == 2 || !a->d_tiles) return vbt_fail(VBT_ERR_INVALID, "the ceiling covers render assets");
+    if (4ull * a->controlsPerFrame > 0xFFFFFFFFull) return vbt_fail(VBT_ERR_INVALID, "frame cache: one frame exceeds 4 GiB");
+    if (a->tileCount[tile] == 0) return VBT_OK;
+    launch_cache_ceiling(tile, a->dev(), a->d_tiles + a->tileOff[tile], a->tileCount[tile], nt, reads, d_out,
+                         a->controlsPerFrame, S(stream));
+    HIPCHK(hipGetLastError());
     return VBT_OK;
 }
 

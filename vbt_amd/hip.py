@@ -181,6 +181,7 @@ def lib():
         L.vbt_asset_autotune.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(c_int), c_void_p]
         L.vbt_tuning_key.argtypes = [c_void_p, c_int, ctypes.c_char_p, ctypes.c_size_t, c_void_p]
+        L.vbt_asset_cache_ceiling.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]
         L.vbt_asset_autotune_table.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(ctypes.c_float),
                                                ctypes.POINTER(c_int), ctypes.POINTER(c_int), c_void_p]
         L.vbt_tuning_load.argtypes = [ctypes.c_char_p]
@@ -461,6 +462,13 @@ class Asset:
         best = c_int()
         check(lib().vbt_asset_autotune(self.h, int(nt or self.dims[3]), reps, ms, ctypes.byref(best), self.stream.h))
         return best.value, [round(float(x), 4) for x in ms]
+
+    def cache_ceiling_into(self, out: DeviceArray, nt: int, tile: int, reads: bool = True,
+                           stream: Stream | None = None):
+        """The frame-cache build's memory traffic without its decode (the
+        layout's ceiling on this asset); `out` gets placeholder values."""
+        check(lib().vbt_asset_cache_ceiling(self.h, int(nt), int(tile), 1 if reads else 0, out.ptr,
+                                            (stream or self.stream).h))
 
     def tuning_key(self, nt: int | None = None) -> str:
         """This asset's key in the tuning table for nt frames per launch."""
