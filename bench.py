@@ -363,7 +363,7 @@ def frames_bench(hip, asset, workload: str, steps: int) -> dict:
     return res
 
 
-def batch_bench(hip, assets: dict, names, steps: int, tune_reps: int = 2) -> dict:
+def batch_bench(hip, assets: dict, names, steps: int, tune_reps: int = 2, fixed=None) -> dict:
     """All frames of each named asset into its packed cache: one replay of
     the batch graph (persistent work-queue kernel, shape autotuned) against
     one plain launch per asset (each with its own autotuned tile)."""
@@ -386,7 +386,13 @@ def batch_bench(hip, assets: dict, names, steps: int, tune_reps: int = 2) -> dic
 
     plain_ms = timed(hip, s, plain, steps)
     batch = hip.CacheBatch(jobs, stream=s)
-    best, tune_ms, trials = batch.autotune(reps=tune_reps)
+    if fixed is None:
+        best, tune_ms, trials = batch.autotune(reps=tune_reps)
+    else:  # a given shape: persistent, tile, ranges, chunk, order, queues, heavyTile
+        sh = hip.default_batch_shape()
+        (sh.persistent, sh.tile, sh.ranges, sh.chunk, sh.order, sh.queues, sh.heavyTile) = fixed
+        batch.shape = sh
+        best, trials = sh.as_dict(), []
     items, wgs = batch.plan_info()
     graph_ms = timed(hip, s, batch.launch, steps)
     direct_ms = timed(hip, s, lambda: batch.launch(graph=False), steps)
@@ -652,6 +658,8 @@ def main():
                     help="assets whose every frame is built in one launch (the Mode2 and scientific kernels)")
     ap.add_argument("--no-frames", action="store_true")
     ap.add_argument("--batch-tune-reps", type=int, default=2)
+    ap.add_argument("--batch-shape", default="",
+                    help="fixed batch shape 'persistent,tile,ranges,chunk,order,queues,heavyTile' (no autotune)")
     ap.add_argument("--batch-assets", default="fire_density,fire_flames,fire_temperature",
                     help="assets of the batched frame-cache build (all frames of each, one graph replay)")
     ap.add_argument("--tile", type=int, default=-1, help="frame-cache tile shape index (default: tuning table, "
@@ -846,9 +854,10 @@ def main():
         line["all_frames"] = {n: frames_bench(hip, extra[n], n, args.steps)
                               for n in [x for x in args.frames_assets.split(",") if x]}
     if not args.no_batch:
-        line["batch_single"] = batch_bench(hip, extra, [args.workload], args.steps, args.batch_tune_reps)
+        fixed = tuple(int(x) for x in args.batch_shape.split(",")) if args.batch_shape else None
+        line["batch_single"] = batch_bench(hip, extra, [args.workload], args.steps, args.batch_tune_reps, fixed)
         names = [x for x in args.batch_assets.split(",") if x]
-        line["batch"] = batch_bench(hip, extra, names, args.steps, args.batch_tune_reps)
+        line["batch"] = batch_bench(hip, extra, names, args.steps, args.batch_tune_reps, fixed)
         bb = line["batch"]
         print(f"[bench] batch {names}: graph {bb['batched_graph']['ms']} ms vs plain "
               f"{bb['plain_per_asset']['ms']} ms ({bb['batched_graph']['decoded_gbs']} GB/s decoded), shape "
