@@ -423,14 +423,17 @@ __global__ void __launch_bounds__(256) k_slot_plan(AssetDev a, uint64_t total, u
 }
 
 template <int LAYOUT, uint32_t BLOCK, uint32_t WINDOW>
-__global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32_t* tileGroup, int t0, int nt,
-                                                        float* out, uint64_t total, uint32_t* errFlag)
+__global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32_t* tileGroup, uint32_t nTiles,
+                                                        uint32_t runLog2, int t0, int nt, float* out, uint64_t total,
+                                                        uint32_t* errFlag)
 {
     __shared__ uint32_t lds[WINDOW / 4 + 4]; // + 4: reads past the window, see sweep_frames
     __shared__ uint32_t sRed[BLOCK / 64];
     if (LAYOUT == VBT_CACHE_PACKED) {
-        // tile = groups [tileGroup[i], tileGroup[i + 1]) (plan_tiles)
-        const uint32_t g0 = tileGroup[blockIdx.x], g1 = tileGroup[blockIdx.x + 1u];
+        // tile i = groups [tileGroup[i], tileGroup[i + 1]) (plan_tiles), in XCD runs
+        const uint32_t tile = xcd_run_tile(blockIdx.x, runLog2);
+        if (tile >= nTiles) return;
+        const uint32_t g0 = tileGroup[tile], g1 = tileGroup[tile + 1u];
         cache_plan_tile<BLOCK, WINDOW>(a, g0, g1 - g0, t0, nt, out, total, errFlag, lds, sRed);
     } else {
         cache_render_tile<LAYOUT, BLOCK, WINDOW>(a, a.groupLeaf, blockIdx.x, t0, nt, out, total, errFlag, lds, sRed);
@@ -445,12 +448,15 @@ __global__ void __launch_bounds__(BLOCK) k_cache_render(AssetDev a, const uint32
 // the ceiling of the build for this asset and layout; what it writes is not a
 // frame cache.
 template <uint32_t BLOCK, uint32_t WINDOW>
-__global__ void __launch_bounds__(BLOCK) k_cache_ceiling(AssetDev a, const uint32_t* tileGroup, int nt, int reads,
-                                                         float* out, uint64_t total)
+__global__ void __launch_bounds__(BLOCK) k_cache_ceiling(AssetDev a, const uint32_t* tileGroup, uint32_t nTiles,
+                                                         uint32_t runLog2, int nt, int reads, float* out,
+                                                         uint64_t total)
 {
     __shared__ uint32_t lds[WINDOW / 4];
     const uint32_t tid = threadIdx.x;
-    const uint32_t g0 = tileGroup[blockIdx.x], ng = tileGroup[blockIdx.x + 1u] - g0;
+    const uint32_t tile = xcd_run_tile(blockIdx.x, runLog2);
+    if (tile >= nTiles) return;
+    const uint32_t g0 = tileGroup[tile], ng = tileGroup[tile + 1u] - g0;
     const uint64_t g = (uint64_t)g0 * CACHE_GROUP + tid;
     const bool live = tid < ng * CACHE_GROUP && g < total;
     uint32_t v = tid;
@@ -700,17 +706,33 @@ __global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_
     }
 }
 
-// Packed layout: grid = the shape's tile list (tileGroup, nTiles entries + 1);
-// 576 layout: one tile per BLOCK slots.
+// runs of 2^r consecutive tiles per XCD (xcd_run_tile): as long as every XCD
+// keeps at least 64 runs, at most 16 tiles a run
+uint32_t tile_run_log2(uint32_t nTiles)
+{
+    uint32_t r = 0;
+    while (r < 4u && (nTiles >> (3u + r + 1u)) >= 64u) ++r;
+    return r;
+}
+
+uint32_t run_grid(uint32_t nTiles, uint32_t runLog2)
+{
+    const uint32_t q = 8u << runLog2;
+    return (uint32_t)(((uint64_t)nTiles + q - 1u) / q * q);
+}
+
+// Packed layout: grid = the shape's tile list (tileGroup, nTiles entries + 1)
+// in XCD runs; 576 layout: one tile per BLOCK slots.
 template <int LAYOUT>
 void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* tileGroup, uint32_t nTiles, int t0, int nt,
                          float* out, uint64_t total, uint32_t* err, hipStream_t s)
 {
+    const uint32_t r = tile_run_log2(nTiles);
     switch (tile) {
 #define VBT_TILE(i, B, W)                                                                                   \
     case i:                                                                                                 \
-        k_cache_render<LAYOUT, B, W><<<LAYOUT == VBT_CACHE_PACKED ? nTiles : blocks_for(total, B), B, 0, s>>>( \
-            ad, tileGroup, t0, nt, out, total, err);                                                        \
+        k_cache_render<LAYOUT, B, W><<<LAYOUT == VBT_CACHE_PACKED ? run_grid(nTiles, r) : blocks_for(total, B), B, \
+                                       0, s>>>(ad, tileGroup, nTiles, r, t0, nt, out, total, err);          \
         break;
         VBT_TILE(0, 64, 8192)
         VBT_TILE(1, 64, 16384)
@@ -731,9 +753,12 @@ void launch_cache_render(int tile, const AssetDev& ad, const uint32_t* tileGroup
 void launch_cache_ceiling(int tile, const AssetDev& ad, const uint32_t* tileGroup, uint32_t nTiles, int nt, int reads,
                           float* out, uint64_t total, hipStream_t s)
 {
+    const uint32_t r = tile_run_log2(nTiles);
     switch (tile) {
 #define VBT_TILE(i, B, W)                                                                                  \
-    case i: k_cache_ceiling<B, W><<<nTiles, B, 0, s>>>(ad, tileGroup, nt, reads, out, total); break;
+    case i:                                                                                                \
+        k_cache_ceiling<B, W><<<run_grid(nTiles, r), B, 0, s>>>(ad, tileGroup, nTiles, r, nt, reads, out, total); \
+        break;
         VBT_TILE(0, 64, 8192)
         VBT_TILE(1, 64, 16384)
         VBT_TILE(2, 128, 16384)

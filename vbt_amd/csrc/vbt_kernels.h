@@ -1051,6 +1051,21 @@ constexpr uint32_t PLAN_SLOW = 0u, PLAN_FP16 = 1u, PLAN_FP16_NAN = 2u, PLAN_FP32
 __host__ __device__ inline bool plan_swept(uint32_t kind) { return kind >= PLAN_FP16 && kind <= PLAN_FP32; }
 __host__ __device__ inline uint32_t plan_kind(uint2 p) { return (p.y >> 8) & 7u; }
 
+// XCD-aware tile order of the one-shot packed build: workgroup b runs on XCD
+// b % 8 (the dispatcher's round robin) and takes the tile
+// ((j >> r) * 8 + b % 8) << r | (j & (2^r - 1)), j = b / 8, so each XCD works
+// through runs of 2^r consecutive tiles.  Neighbouring tiles share the
+// 128-byte lines at the ends of their row segments (a frame's row starts at
+// any 4-byte offset); written from one XCD the two halves meet in its L2
+// before the line goes to memory, from two XCDs each writes a partial line.
+// The grid is rounded up to a multiple of 8 << r; tiles past the list are
+// skipped.
+__device__ __forceinline__ uint32_t xcd_run_tile(uint32_t b, uint32_t runLog2)
+{
+    const uint32_t j = b >> 3;
+    return ((((j >> runLog2) << 3) + (b & 7u)) << runLog2) | (j & ((1u << runLog2) - 1u));
+}
+
 // Wave-uniform loads from tables that no kernel writes while it runs (tile
 // lists, group ranges, batch work items: written at asset open or before the
 // launch).  Read through the constant address space they stay scalar loads
