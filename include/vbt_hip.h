@@ -345,7 +345,9 @@ typedef struct {
     int queues;  /* 1: one queue; 8: one queue per XCD (workgroup i starts on queue i % 8, then steals) */
     int wgPerCU; /* persistent workgroups per CU; 0: as many as fit */
     int persistent; /* 1: persistent workgroups claim items from the queues; 0: one workgroup per `chunk`
-                       items in queue order (static) */
+                       items in queue order (static); 2: no queue, each job's own one-shot build
+                       (vbt_decode_frame_cache's kernels at its asset's tile shape), the jobs alternating
+                       between two graph branches */
     int heavyTile;  /* static form only: tiles whose keyframe bytes exceed the tile's LDS window run in a
                        second, concurrent kernel with this tile shape (same slots, larger window); -1: none */
 } vbt_batch_shape;

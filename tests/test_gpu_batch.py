@@ -164,6 +164,7 @@ def test_one_shot_kernel_every_tile_shape(world, tile):
     dict(persistent=0, ranges=7, chunk=3, order=0),
     dict(persistent=0, tile=6, heavyTile=1, order=1),             # 64 slots: 4 KiB light, 16 KiB heavy window
     dict(persistent=0, tile=7, heavyTile=3, ranges=2, chunk=2),   # 128 slots: 8 KiB light, 32 KiB heavy
+    dict(persistent=2),                                           # each job's one-shot build, two branches
 ], ids=lambda kw: "-".join(f"{k}{v}" for k, v in kw.items()))
 def test_queue_shapes_and_replays(world, kw):
     """Every replay of the captured graph is checked, with eager decode work
@@ -172,7 +173,7 @@ def test_queue_shapes_and_replays(world, kw):
     assets = world[0]
     side = hip.DeviceArray((2, assets["smoke64"][0].info.controlsPerFrame), np.float32)
     items, wgs = b.plan_info()
-    assert items > 0 and 0 < wgs <= items
+    assert items > 0 and (0 < wgs <= items or (kw.get("persistent") == 2 and wgs == 0))
     for rep in range(3):
         for o in outs:
             o.fill_bytes(0xFF)
@@ -250,7 +251,8 @@ def test_invalid_jobs_and_shapes(world):
         hip.CacheBatch([(a, 1, F, hip.CACHE_PACKED, out)])
     with pytest.raises(hip.VbtError, match="packed cache only"):
         hip.CacheBatch([(s, 0, 1, hip.CACHE_STRIDE576, out)])
-    for kw in [dict(tile=10), dict(ranges=0), dict(chunk=65), dict(order=2), dict(queues=9), dict(persistent=2),
+    for kw in [dict(tile=10), dict(ranges=0), dict(chunk=65), dict(order=2), dict(queues=9), dict(persistent=3),
+               dict(persistent=2, tile=0, heavyTile=1),    # no heavy kernel in the per-job form
                dict(heavyTile=1),                          # heavy tiles need the static form
                dict(persistent=0, tile=0, heavyTile=2),    # different slots per workgroup
                dict(persistent=0, tile=1, heavyTile=0)]:   # smaller window

@@ -101,6 +101,10 @@ def test_two_runs_match_the_batch(assets):
     second = run_both(assets, small, shape(tile=0, ranges=1, chunk=1, queues=1, persistent=0))
     for (name, t0, nt, layout), (got, exp) in zip(small, second):
         bitexact(got, exp, f"second run: {name} [{t0},{t0 + nt}) layout {layout}")
+    # the per-job form (each job's one-shot build) uses no queue at all
+    third = run_both(assets, small, shape(tile=0, persistent=2))
+    for (name, t0, nt, layout), (got, exp) in zip(small, third):
+        bitexact(got, exp, f"per-job form: {name} [{t0},{t0 + nt}) layout {layout}")
 
 
 def test_default_shape_and_scientific_only(assets):

@@ -307,7 +307,9 @@ int check_shape(const vbt_batch_shape& s)
     if (s.order != 0 && s.order != 1) return vbt_fail(VBT_ERR_INVALID, "batch shape: order must be 0 or 1");
     if (s.queues < 1 || s.queues > (int)MAX_QUEUES) return vbt_fail(VBT_ERR_INVALID, "batch shape: queues out of range");
     if (s.wgPerCU < 0 || s.wgPerCU > 64) return vbt_fail(VBT_ERR_INVALID, "batch shape: wgPerCU out of range");
-    if (s.persistent != 0 && s.persistent != 1) return vbt_fail(VBT_ERR_INVALID, "batch shape: persistent must be 0 or 1");
+    if (s.persistent < 0 || s.persistent > 2) return vbt_fail(VBT_ERR_INVALID, "batch shape: persistent must be 0, 1 or 2");
+    if (s.persistent == 2 && s.heavyTile >= 0)
+        return vbt_fail(VBT_ERR_INVALID, "batch shape: the per-job form has no heavy-tile kernel");
     if (s.heavyTile < -1 || s.heavyTile >= CACHE_TILE_COUNT)
         return vbt_fail(VBT_ERR_INVALID, "batch shape: heavyTile out of range");
     if (s.heavyTile >= 0 && (s.persistent || CACHE_TILES[s.heavyTile].block != CACHE_TILES[s.tile].block ||
@@ -421,6 +423,7 @@ int plan(vbt_cache_batch b, const vbt_batch_shape& shape)
     b->items = (uint64_t)heavyBase * R;
     const uint64_t claims = (b->items + (uint64_t)shape.chunk - 1u) / (uint64_t)shape.chunk;
     b->grid = shape.persistent ? (uint32_t)std::min<uint64_t>((uint64_t)b->numCU * perCU, claims) : (uint32_t)claims;
+    if (shape.persistent == 2) b->grid = 0; // per-job one-shot kernels instead of the work queue
     b->heavyBase = (uint32_t)heavyBase;
     b->heavyItems = (uint64_t)heavy.size() * R;
     b->heavyGrid = (uint32_t)((b->heavyItems + (uint64_t)shape.chunk - 1u) / (uint64_t)shape.chunk);
@@ -428,8 +431,34 @@ int plan(vbt_cache_batch b, const vbt_batch_shape& shape)
     return VBT_OK;
 }
 
+// Per-job form: each render job's own one-shot build (its asset's tile shape,
+// XCD runs), the jobs alternating between the stream and a second branch so
+// that one job's tail overlaps the next job's start; scientific jobs on a
+// third branch as in the queue form.
+int enqueue_per_job(vbt_cache_batch b, hipStream_t s)
+{
+    HIPCHK(hipEventRecord(b->evFork, s));
+    HIPCHK(hipStreamWaitEvent(b->side, b->evFork, 0));
+    HIPCHK(hipStreamWaitEvent(b->side2, b->evFork, 0));
+    int k = 0;
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        const vbt_cache_job& job = b->jobs[j];
+        const bool sci = b->kinds[j] == JOB_SCI;
+        hipStream_t st = sci ? b->side : ((k++ & 1) ? b->side2 : s);
+        const int rc = vbt_internal_frame_cache_plain(job.asset, job.t0, job.nt,
+                                                      sci ? VBT_CACHE_PACKED : job.layout, job.d_out, st);
+        if (rc != VBT_OK && rc != VBT_ERR_FORMAT) return rc; // poisoned leaves are reported after the launch
+    }
+    HIPCHK(hipEventRecord(b->evJoin, b->side));
+    HIPCHK(hipEventRecord(b->evJoin2, b->side2));
+    HIPCHK(hipStreamWaitEvent(s, b->evJoin, 0));
+    HIPCHK(hipStreamWaitEvent(s, b->evJoin2, 0));
+    return VBT_OK;
+}
+
 int enqueue(vbt_cache_batch b, hipStream_t s)
 {
+    if (b->shape.persistent == 2) return enqueue_per_job(b, s);
     k_zero_u32<<<1, 256, 0, s>>>(b->d_counters, MAX_QUEUES * COUNTER_STRIDE);
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         if (b->kinds[j] != JOB_576) continue;
@@ -711,6 +740,12 @@ int vbt_cache_batch_autotune(vbt_cache_batch b, int reps, vbt_batch_shape* best,
             sh.tile = t;
             rc = timeShape(sh);
         }
+    if (!rc) { // each job's own one-shot build, jobs on two branches
+        vbt_batch_shape sh = cur;
+        sh.persistent = 2;
+        sh.heavyTile = -1;
+        rc = timeShape(sh);
+    }
     for (int form : {1, 0}) {
         if (rc) break;
         vbt_batch_shape lt = top;
@@ -918,6 +953,19 @@ int vbt_cache_batch_run_noalloc(const vbt_cache_job* jobs, int nJobs, const vbt_
     NaDevice nd;
     if ((rc = na_scratch(dev, nd))) return rc;
     hipStream_t s = (hipStream_t)stream;
+    if (sh.persistent == 2) { // each job's own one-shot build: no work queue, nothing in the scratch array
+        uint32_t kind;
+        for (int j = 0; j < nJobs; ++j)
+            if ((rc = check_job(jobs[j], dev, kind))) return rc;
+        bool bad = false;
+        for (int j = 0; j < nJobs; ++j) {
+            const vbt_cache_job& job = jobs[j];
+            rc = vbt_internal_frame_cache_plain(job.asset, job.t0, job.nt, job.layout, job.d_out, s);
+            if (rc == VBT_ERR_FORMAT) bad = true;
+            else if (rc) return rc;
+        }
+        return bad ? VBT_ERR_FORMAT : VBT_OK;
+    }
     const uint32_t BLOCK = CACHE_TILES[sh.tile].block, R = (uint32_t)sh.ranges;
     BatchJob hj[NA_MAX_JOBS];
     uint32_t kinds[NA_MAX_JOBS];

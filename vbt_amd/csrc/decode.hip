@@ -1764,7 +1764,14 @@ int vbt_asset_leaf_status(vbt_asset a, uint32_t* d_status, void* stream)
     return VBT_OK;
 }
 
+static int frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out, void* stream, bool useIndex);
+
 int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out, void* stream)
+{
+    return frame_cache(a, t0, nt, layout, d_out, stream, true);
+}
+
+static int frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out, void* stream, bool useIndex)
 {
     if (!a || !d_out || nt <= 0) return vbt_fail(VBT_ERR_INVALID, "invalid frame cache request");
     if (t0 < 0 || (uint64_t)t0 + (uint64_t)nt > a->h.frames)
@@ -1782,7 +1789,7 @@ int vbt_decode_frame_cache(vbt_asset a, int t0, int nt, int layout, float* d_out
         launch_sci_frames(a->A, a->sciKeepMax, a->modeCounts[2] + a->modeCounts[3] > 0, ad, t0, nt, d_out,
                           a->controlsPerFrame, a->d_err, s);
     } else if (layout == VBT_CACHE_PACKED) {
-        if (nt <= CACHE_EXACT_MAX_FRAMES && a->index.off) {
+        if (nt <= CACHE_EXACT_MAX_FRAMES && useIndex && a->index.off) {
             launch_indexed(a, a->index, t0, nt, d_out, s);
             HIPCHK(hipGetLastError());
             int rc = note_index_use(a, s);
@@ -2111,3 +2118,10 @@ int vbt_event_elapsed_ms(void* a, void* b, float* ms)
 }
 
 } // extern "C"
+
+// the build without the frame index (no event records: batch graphs capture it)
+int vbt_internal_frame_cache_plain(vbt_asset a, int t0, int nt, int layout, float* d_out, void* stream)
+{
+    return frame_cache(a, t0, nt, layout, d_out, stream, false);
+}
+

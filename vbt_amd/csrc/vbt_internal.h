@@ -25,6 +25,9 @@ const vbt_file_header* vbt_internal_asset_header(vbt_asset a);
 // device list returned, the host copy in *host (count + 1 entries).  Scientific
 // assets have none (count 0, nullptr).
 const uint32_t* vbt_internal_asset_tiles(vbt_asset a, int shape, uint32_t* count, const uint32_t** host);
+// vbt_decode_frame_cache without the frame index (no event records, so a
+// stream capture can hold it)
+int vbt_internal_frame_cache_plain(vbt_asset a, int t0, int nt, int layout, float* d_out, void* stream);
 
 // A frame index of one asset: per bin of the frame range, each packed-cache
 // slot's run of keyframe records (offsets bins x (slots + 1), then the
