@@ -210,14 +210,19 @@ __global__ void __launch_bounds__(BLOCK) k_cache_batch(BatchArgs A)
     }
 }
 
-// Static form of the same queue: workgroup i takes items i*chunk ..
-// i*chunk+chunk-1 of the queue order (the hardware dispatcher balances).
+// Static form of the same queue: the claim c = xcd_run_tile(i) of workgroup
+// i takes items c*chunk .. c*chunk+chunk-1 of the queue order (the hardware
+// dispatcher balances; each XCD works through runs of consecutive claims, so
+// neighbouring tiles' shared row-end lines meet in one L2).
 template <uint32_t BLOCK, uint32_t WINDOW>
 __global__ void __launch_bounds__(BLOCK) k_cache_batch_static(BatchArgs A)
 {
     __shared__ uint32_t lds[WINDOW / 4 + 4]; // + 4: reads past the window, see sweep_frames
     __shared__ uint32_t sRed[BLOCK / 64];
-    const uint32_t first = blockIdx.x * A.chunk;
+    const uint32_t claims = (A.items + A.chunk - 1u) / A.chunk;
+    const uint32_t c = xcd_run_tile(blockIdx.x, tile_run_log2(claims));
+    if (c >= claims) return;
+    const uint32_t first = c * A.chunk;
     const uint32_t end = min(first + A.chunk, A.items);
     for (uint32_t it = first; it < end; ++it) batch_item<BLOCK, WINDOW>(A, it, lds, sRed);
 }
@@ -249,7 +254,7 @@ void launch_batch(int tile, bool persistent, uint32_t grid, const BatchArgs& A, 
 #define VBT_BATCH(i, B, W)                                                                                    \
     case i:                                                                                                   \
         if (persistent) k_cache_batch<B, W><<<grid, B, 0, s>>>(A);                                            \
-        else k_cache_batch_static<B, W><<<grid, B, 0, s>>>(A);                                                \
+        else k_cache_batch_static<B, W><<<run_grid(grid, tile_run_log2(grid)), B, 0, s>>>(A);                                                \
         break;
         VBT_BATCH(0, 64, 8192)
         VBT_BATCH(1, 64, 16384)

@@ -706,21 +706,6 @@ __global__ void __launch_bounds__(256) k_cache_indexed(AssetDev a, const uint32_
     }
 }
 
-// runs of 2^r consecutive tiles per XCD (xcd_run_tile): as long as every XCD
-// keeps at least 64 runs, at most 16 tiles a run
-uint32_t tile_run_log2(uint32_t nTiles)
-{
-    uint32_t r = 0;
-    while (r < 4u && (nTiles >> (3u + r + 1u)) >= 64u) ++r;
-    return r;
-}
-
-uint32_t run_grid(uint32_t nTiles, uint32_t runLog2)
-{
-    const uint32_t q = 8u << runLog2;
-    return (uint32_t)(((uint64_t)nTiles + q - 1u) / q * q);
-}
-
 // Packed layout: grid = the shape's tile list (tileGroup, nTiles entries + 1)
 // in XCD runs; 576 layout: one tile per BLOCK slots.
 template <int LAYOUT>

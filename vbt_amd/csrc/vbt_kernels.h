@@ -1066,6 +1066,22 @@ __device__ __forceinline__ uint32_t xcd_run_tile(uint32_t b, uint32_t runLog2)
     return ((((j >> runLog2) << 3) + (b & 7u)) << runLog2) | (j & ((1u << runLog2) - 1u));
 }
 
+// r of the runs over n tiles (or work items): as long as every XCD keeps at
+// least 64 runs, at most 16 tiles a run
+__host__ __device__ inline uint32_t tile_run_log2(uint32_t n)
+{
+    uint32_t r = 0;
+    while (r < 4u && (n >> (3u + r + 1u)) >= 64u) ++r;
+    return r;
+}
+
+// the grid of n tiles in runs of 2^r: a multiple of 8 << r
+__host__ __device__ inline uint32_t run_grid(uint32_t n, uint32_t runLog2)
+{
+    const uint32_t q = 8u << runLog2;
+    return (uint32_t)(((uint64_t)n + q - 1u) / q * q);
+}
+
 // Wave-uniform loads from tables that no kernel writes while it runs (tile
 // lists, group ranges, batch work items: written at asset open or before the
 // launch).  Read through the constant address space they stay scalar loads
