@@ -442,9 +442,12 @@ int plan(vbt_cache_batch b, const vbt_batch_shape& shape)
 // third branch as in the queue form.
 int enqueue_per_job(vbt_cache_batch b, hipStream_t s)
 {
-    HIPCHK(hipEventRecord(b->evFork, s));
-    HIPCHK(hipStreamWaitEvent(b->side, b->evFork, 0));
-    HIPCHK(hipStreamWaitEvent(b->side2, b->evFork, 0));
+    // branches only where there is work for them (a branch costs a fork and a join)
+    const uint32_t render = (uint32_t)b->jobs.size() - b->sciJobs;
+    const bool useSide = b->sciJobs > 0, useSide2 = render > 1;
+    if (useSide || useSide2) HIPCHK(hipEventRecord(b->evFork, s));
+    if (useSide) HIPCHK(hipStreamWaitEvent(b->side, b->evFork, 0));
+    if (useSide2) HIPCHK(hipStreamWaitEvent(b->side2, b->evFork, 0));
     int k = 0;
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         const vbt_cache_job& job = b->jobs[j];
@@ -454,10 +457,14 @@ int enqueue_per_job(vbt_cache_batch b, hipStream_t s)
                                                       sci ? VBT_CACHE_PACKED : job.layout, job.d_out, st);
         if (rc != VBT_OK && rc != VBT_ERR_FORMAT) return rc; // poisoned leaves are reported after the launch
     }
-    HIPCHK(hipEventRecord(b->evJoin, b->side));
-    HIPCHK(hipEventRecord(b->evJoin2, b->side2));
-    HIPCHK(hipStreamWaitEvent(s, b->evJoin, 0));
-    HIPCHK(hipStreamWaitEvent(s, b->evJoin2, 0));
+    if (useSide) {
+        HIPCHK(hipEventRecord(b->evJoin, b->side));
+        HIPCHK(hipStreamWaitEvent(s, b->evJoin, 0));
+    }
+    if (useSide2) {
+        HIPCHK(hipEventRecord(b->evJoin2, b->side2));
+        HIPCHK(hipStreamWaitEvent(s, b->evJoin2, 0));
+    }
     return VBT_OK;
 }
 
